@@ -1,0 +1,10 @@
+// Exact arithmetic instantiation.  Built with -ffp-contract=off and IEEE
+// division / square root, so every operation rounds as an IEEE float32
+// evaluation of the reference expressions would.
+#include "fv_launch.h"
+
+namespace fvmi {
+hipError_t launch_substep_exact(int scheme, const SubstepArgs& a, hipStream_t stream) {
+    return launch_substep<MathExact>(scheme, a, stream);
+}
+}  // namespace fvmi

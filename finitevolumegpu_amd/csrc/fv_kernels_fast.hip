@@ -1,0 +1,9 @@
+// Fast arithmetic instantiation (hardware reciprocal / square root, contraction
+// allowed).  The production path.
+#include "fv_launch.h"
+
+namespace fvmi {
+hipError_t launch_substep_fast(int scheme, const SubstepArgs& a, hipStream_t stream) {
+    return launch_substep<MathFast>(scheme, a, stream);
+}
+}  // namespace fvmi

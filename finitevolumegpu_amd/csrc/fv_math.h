@@ -1,0 +1,376 @@
+// Device arithmetic policies, limiters and fluxes of the four dimensionally
+// split schemes.
+//
+// Every function is written in "sweep order": state index 0 is the density
+// (h or rho), 1 the momentum normal to the sweep, 2 the tangential momentum,
+// 3 the energy (Euler).  A y sweep passes (h, hv, hu[, E]) and maps the result
+// back, as computeFluxG does in GPUSimulators/cuda/SWE2D_KP07_dimsplit.cu:71-110.
+//
+// Two arithmetic policies share the code:
+//   MathExact  IEEE division and square root, and (by the translation unit's
+//              -ffp-contract=off) no fused multiply-add.  Each expression is
+//              evaluated in the operand order of the CUDA source, so results
+//              are bitwise equal to an IEEE float32 evaluation of the
+//              reference arithmetic.
+//   MathFast   hardware reciprocal / square root (v_rcp_f32, v_sqrt_f32, about
+//              1 ulp) with shared reciprocals and contraction allowed.  Judged
+//              against the float32 oracle at relative L1 <= 1e-5.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace fvmi {
+
+struct MathExact {
+    static constexpr bool exact = true;
+    __device__ static __forceinline__ float div(float a, float b) { return a / b; }
+    __device__ static __forceinline__ float sqrt(float x) { return __builtin_sqrtf(x); }
+    __device__ static __forceinline__ float rcp(float x) { return 1.0f / x; }
+};
+
+struct MathFast {
+    static constexpr bool exact = false;
+    __device__ static __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+    __device__ static __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    __device__ static __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+};
+
+// ---------------------------------------------------------------------------
+// Limiters: GPUSimulators/cuda/limiters.h, common.h
+// ---------------------------------------------------------------------------
+
+// minmodSlope, limiters.h:31-41
+template <class M>
+__device__ __forceinline__ float minmod_slope(float left, float center, float right, float theta) {
+    const float backward = (center - left) * theta;
+    const float central = (right - left) * 0.5f;
+    const float forward = (right - center) * theta;
+    if constexpr (M::exact) {
+        const float sb = __builtin_copysignf(1.0f, backward);
+        const float sc = __builtin_copysignf(1.0f, central);
+        const float sf = __builtin_copysignf(1.0f, forward);
+        return 0.25f * sb * (sb + sc) * (sc + sf) *
+               fminf(fminf(fabsf(backward), fabsf(central)), fabsf(forward));
+    } else {
+        // Same value: minmod(b, c, f) = med3(med3(b, f, 0), c, 0) -- the inner
+        // median is 0 unless b and f share a sign, and then c shares it too.
+        return __builtin_amdgcn_fmed3f(__builtin_amdgcn_fmed3f(backward, forward, 0.0f), central, 0.0f);
+    }
+}
+
+// generalized_minmod, limiters.h:100-102
+__device__ __forceinline__ float generalized_minmod(float r, float theta) {
+    return fmaxf(0.0f, fminf(theta * r, fminf((1.0f + r) / 2.0f, theta)));
+}
+
+// desingularize, common.h:81-83
+template <class M>
+__device__ __forceinline__ float desingularize(float x, float eps) {
+    return __builtin_copysignf(1.0f, x) *
+           fmaxf(fabsf(x), fminf(M::div(x * x, 2.0f * eps) + 0.5f * eps, eps));
+}
+
+// limiterToWAFLimiter, SWECommon.h:109-111
+__device__ __forceinline__ float limiter_to_waf(float r, float c) {
+    return 1.0f - (1.0f - fabsf(c)) * r;
+}
+
+// ---------------------------------------------------------------------------
+// Shallow water: GPUSimulators/cuda/SWECommon.h
+// ---------------------------------------------------------------------------
+
+// F_func, SWECommon.h:29-37
+template <class M>
+__device__ __forceinline__ void swe_F(const float q[3], float g, float F[3]) {
+    if constexpr (M::exact) {
+        F[0] = q[1];
+        F[1] = q[1] * q[1] / q[0] + 0.5f * g * q[0] * q[0];
+        F[2] = q[1] * q[2] / q[0];
+    } else {
+        const float u = q[1] * M::rcp(q[0]);
+        F[0] = q[1];
+        F[1] = q[1] * u + 0.5f * g * q[0] * q[0];
+        F[2] = q[2] * u;
+    }
+}
+
+// CentralUpwindFlux, SWECommon.h:228-241.  qm = left state, qp = right state.
+template <class M>
+__device__ __forceinline__ void swe_cu_flux(const float qm[3], const float qp[3], float g, float out[3]) {
+    if constexpr (M::exact) {
+        float Fp[3], Fm[3];
+        swe_F<M>(qp, g, Fp);
+        const float up = qp[1] / qp[0];
+        const float cp = M::sqrt(g * qp[0]);
+        swe_F<M>(qm, g, Fm);
+        const float um = qm[1] / qm[0];
+        const float cm = M::sqrt(g * qm[0]);
+        const float am = fminf(fminf(um - cm, up - cp), 0.0f);
+        const float ap = fmaxf(fmaxf(um + cm, up + cp), 0.0f);
+        const float apam = ap * am;
+        const float den = ap - am;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) out[k] = ((ap * Fm[k] - am * Fp[k]) + apam * (qp[k] - qm[k])) / den;
+    } else {
+        const float ip = M::rcp(qp[0]);
+        const float im = M::rcp(qm[0]);
+        const float up = qp[1] * ip, um = qm[1] * im;
+        const float cp = M::sqrt(g * qp[0]), cm = M::sqrt(g * qm[0]);
+        const float hg = 0.5f * g;
+        const float Fp[3] = {qp[1], qp[1] * up + hg * qp[0] * qp[0], qp[2] * up};
+        const float Fm[3] = {qm[1], qm[1] * um + hg * qm[0] * qm[0], qm[2] * um};
+        const float am = fminf(fminf(um - cm, up - cp), 0.0f);
+        const float ap = fmaxf(fmaxf(um + cm, up + cp), 0.0f);
+        const float apam = ap * am;
+        const float iden = M::rcp(ap - am);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) out[k] = ((ap * Fm[k] - am * Fp[k]) + apam * (qp[k] - qm[k])) * iden;
+    }
+}
+
+// HLL_flux (SWE), SWECommon.h:274-313
+template <class M>
+__device__ __forceinline__ void swe_hll_flux(const float ql[3], const float qr[3], float g, float out[3]) {
+    const float h_l = ql[0], h_r = qr[0];
+    float u_l, u_r;
+    if constexpr (M::exact) {
+        u_l = ql[1] / h_l;
+        u_r = qr[1] / h_r;
+    } else {
+        u_l = ql[1] * M::rcp(h_l);
+        u_r = qr[1] * M::rcp(h_r);
+    }
+    const float c_l = M::sqrt(g * h_l);
+    const float c_r = M::sqrt(g * h_r);
+    const float h_dag = 0.5f * (h_l + h_r) - M::div(0.25f * (u_r - u_l) * (h_l + h_r), c_l + c_r);
+    const float q_l_tmp = M::sqrt(0.5f * M::div((h_dag + h_l) * h_dag, h_l * h_l));
+    const float q_r_tmp = M::sqrt(0.5f * M::div((h_dag + h_r) * h_dag, h_r * h_r));
+    const float q_l = (h_dag > h_l) ? q_l_tmp : 1.0f;
+    const float q_r = (h_dag > h_r) ? q_r_tmp : 1.0f;
+    const float S_l = u_l - c_l * q_l;
+    const float S_r = u_r + c_r * q_r;
+    float F_l[3], F_r[3];
+    swe_F<M>(ql, g, F_l);
+    swe_F<M>(qr, g, F_r);
+    const float srsl = S_r * S_l;
+    if constexpr (M::exact) {
+        const float den = S_r - S_l;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float star = (S_r * F_l[k] - S_l * F_r[k] + srsl * (qr[k] - ql[k])) / den;
+            out[k] = (S_l >= 0.0f) ? F_l[k] : ((S_r <= 0.0f) ? F_r[k] : star);
+        }
+    } else {
+        const float iden = M::rcp(S_r - S_l);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float star = (S_r * F_l[k] - S_l * F_r[k] + srsl * (qr[k] - ql[k])) * iden;
+            out[k] = (S_l >= 0.0f) ? F_l[k] : ((S_r <= 0.0f) ? F_r[k] : star);
+        }
+    }
+}
+
+// computeHStar, SWECommon.h:114-121
+template <class M>
+__device__ __forceinline__ float compute_h_star(float h_l, float h_r, float u_l, float u_r, float c_l, float c_r,
+                                                float g) {
+    (void)h_l;
+    (void)h_r;
+    const float h_tmp = 0.5f * (c_l + c_r) + 0.25f * (u_l - u_r);
+    return M::div(h_tmp * h_tmp, g);
+}
+
+// WAF_1D_flux, SWECommon.h:131-210: interface between q_l1 and q_r1
+template <class M>
+__device__ __forceinline__ void swe_waf_flux(const float q_l2[3], const float q_l1[3], const float q_r1[3],
+                                             const float q_r2[3], float g, float dx, float dt, float out[3]) {
+    const float h_l = q_l1[0], h_r = q_r1[0];
+    const float h_l2 = q_l2[0], h_r2 = q_r2[0];
+    float u_l, u_r, u_l2, u_r2, v_l, v_r, v_l2, v_r2;
+    if constexpr (M::exact) {
+        u_l = q_l1[1] / h_l;
+        u_r = q_r1[1] / h_r;
+        u_l2 = q_l2[1] / h_l2;
+        u_r2 = q_r2[1] / h_r2;
+        v_l = q_l1[2] / h_l;
+        v_r = q_r1[2] / h_r;
+        v_l2 = q_l2[2] / h_l2;
+        v_r2 = q_r2[2] / h_r2;
+    } else {
+        const float il = M::rcp(h_l), ir = M::rcp(h_r), il2 = M::rcp(h_l2), ir2 = M::rcp(h_r2);
+        u_l = q_l1[1] * il;
+        u_r = q_r1[1] * ir;
+        u_l2 = q_l2[1] * il2;
+        u_r2 = q_r2[1] * ir2;
+        v_l = q_l1[2] * il;
+        v_r = q_r1[2] * ir;
+        v_l2 = q_l2[2] * il2;
+        v_r2 = q_r2[2] * ir2;
+    }
+    const float c_l = M::sqrt(g * h_l);
+    const float c_r = M::sqrt(g * h_r);
+    const float c_l2 = M::sqrt(g * h_l2);
+    const float c_r2 = M::sqrt(g * h_r2);
+    const float h_dag_l = compute_h_star<M>(h_l2, h_l, u_l2, u_l, c_l2, c_l, g);
+    const float h_dag = compute_h_star<M>(h_l, h_r, u_l, u_r, c_l, c_r, g);
+    const float h_dag_r = compute_h_star<M>(h_r, h_r2, u_r, u_r2, c_r, c_r2, g);
+    const float q_l_tmp = M::div(M::sqrt(0.5f * ((h_dag + h_l) * h_dag)), h_l);
+    const float q_r_tmp = M::div(M::sqrt(0.5f * ((h_dag + h_r) * h_dag)), h_r);
+    const float q_l = (h_dag > h_l) ? q_l_tmp : 1.0f;
+    const float q_r = (h_dag > h_r) ? q_r_tmp : 1.0f;
+    const float S_l = u_l - c_l * q_l;
+    const float S_r = u_r + c_r * q_r;
+    const float S_star = M::div(S_l * h_r * (u_r - S_r) - S_r * h_l * (u_l - S_l),
+                                h_r * (u_r - S_r) - h_l * (u_l - S_l));
+    const float s_l = M::div(h_l * (S_l - u_l), S_l - S_star);
+    const float s_r = M::div(h_r * (S_r - u_r), S_r - S_star);
+    const float Q_star_l[3] = {s_l * 1.0f, s_l * S_star, s_l * v_l};
+    const float Q_star_r[3] = {s_r * 1.0f, s_r * S_star, s_r * v_r};
+    float F_1[3], F_4[3], F_2[3], F_3[3];
+    swe_F<M>(q_l1, g, F_1);
+    swe_F<M>(q_r1, g, F_4);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        F_2[k] = F_1[k] + S_l * (Q_star_l[k] - q_l1[k]);
+        F_3[k] = F_4[k] + S_r * (Q_star_r[k] - q_r1[k]);
+    }
+    float c_1, c_2, c_3;
+    if constexpr (M::exact) {
+        c_1 = S_l * dt / dx;
+        c_2 = S_star * dt / dx;
+        c_3 = S_r * dt / dx;
+    } else {
+        const float k = dt * M::rcp(dx);
+        c_1 = S_l * k;
+        c_2 = S_star * k;
+        c_3 = S_r * k;
+    }
+    const float eps = 1.0e-6f;
+    const float r_1 = M::div(desingularize<M>((c_1 > 0.0f) ? (h_dag_l - h_l2) : (h_dag_r - h_r), eps),
+                             desingularize<M>(h_dag - h_l, eps));
+    const float r_2 = M::div(desingularize<M>((c_2 > 0.0f) ? (v_l - v_l2) : (v_r2 - v_r), eps),
+                             desingularize<M>(v_r - v_l, eps));
+    const float r_3 = M::div(desingularize<M>((c_3 > 0.0f) ? (h_l - h_dag_l) : (h_r2 - h_dag_r), eps),
+                             desingularize<M>(h_r - h_dag, eps));
+    const float A_1 = __builtin_copysignf(1.0f, c_1) * limiter_to_waf(generalized_minmod(r_1, 1.9f), c_1);
+    const float A_2 = __builtin_copysignf(1.0f, c_2) * limiter_to_waf(generalized_minmod(r_2, 1.9f), c_2);
+    const float A_3 = __builtin_copysignf(1.0f, c_3) * limiter_to_waf(generalized_minmod(r_3, 1.9f), c_3);
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        out[k] = 0.5f * (F_1[k] + F_4[k]) -
+                 0.5f * (A_1 * (F_2[k] - F_1[k]) + A_2 * (F_3[k] - F_2[k]) + A_3 * (F_4[k] - F_3[k]));
+}
+
+// writeCfl per-cell value, SWECommon.h:497-504
+template <class M>
+__device__ __forceinline__ float swe_cell_dt(const float q[3], float dx, float dy, float g) {
+    if constexpr (M::exact) {
+        const float h = q[0];
+        const float u = q[1] / h;
+        const float v = q[2] / h;
+        const float max_u = dx / (fabsf(u) + M::sqrt(g * h));
+        const float max_v = dy / (fabsf(v) + M::sqrt(g * h));
+        return fminf(max_u, max_v);
+    } else {
+        const float ih = M::rcp(q[0]);
+        const float c = M::sqrt(g * q[0]);
+        return fminf(M::div(dx, fabsf(q[1] * ih) + c), M::div(dy, fabsf(q[2] * ih) + c));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Euler: GPUSimulators/cuda/EulerCommon.h
+// ---------------------------------------------------------------------------
+
+// pressure, EulerCommon.h:83-90
+template <class M>
+__device__ __forceinline__ float euler_pressure(const float q[4], float gamma) {
+    return (gamma - 1.0f) * (q[3] - M::div(0.5f * (q[1] * q[1] + q[2] * q[2]), q[0]));
+}
+
+// F_func, EulerCommon.h:93-109.  P is a pressure.
+template <class M>
+__device__ __forceinline__ void euler_F(const float q[4], float P, float F[4]) {
+    const float u = M::div(q[1], q[0]);
+    F[0] = q[1];
+    F[1] = q[1] * u + P;
+    F[2] = q[2] * u;
+    F[3] = u * (q[3] + P);
+}
+
+// HLL_flux (Euler), EulerCommon.h:117-160
+template <class M>
+__device__ __forceinline__ void euler_hll_flux(const float ql[4], const float qr[4], float gamma, float out[4]) {
+    const float h_l = ql[0], h_r = qr[0];
+    const float u_l = M::div(ql[1], h_l);
+    const float u_r = M::div(qr[1], h_r);
+    const float P_l = euler_pressure<M>(ql, gamma);
+    const float P_r = euler_pressure<M>(qr, gamma);
+    const float c_l = M::sqrt(M::div(gamma * P_l, ql[0]));
+    const float c_r = M::sqrt(M::div(gamma * P_r, qr[0]));
+    const float h_dag = 0.5f * (h_l + h_r) - M::div(0.25f * (u_r - u_l) * (h_l + h_r), c_l + c_r);
+    const float q_l_tmp = M::sqrt(0.5f * M::div((h_dag + h_l) * h_dag, h_l * h_l));
+    const float q_r_tmp = M::sqrt(0.5f * M::div((h_dag + h_r) * h_dag, h_r * h_r));
+    const float q_l = (h_dag > h_l) ? q_l_tmp : 1.0f;
+    const float q_r = (h_dag > h_r) ? q_r_tmp : 1.0f;
+    const float S_l = u_l - c_l * q_l;
+    const float S_r = u_r + c_r * q_r;
+    float F_l[4], F_r[4];
+    euler_F<M>(ql, P_l, F_l);
+    euler_F<M>(qr, P_r, F_r);
+    const float srsl = S_r * S_l;
+    if constexpr (M::exact) {
+        const float den = S_r - S_l;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float star = (S_r * F_l[k] - S_l * F_r[k] + srsl * (qr[k] - ql[k])) / den;
+            out[k] = (S_l >= 0.0f) ? F_l[k] : ((S_r <= 0.0f) ? F_r[k] : star);
+        }
+    } else {
+        const float iden = M::rcp(S_r - S_l);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float star = (S_r * F_l[k] - S_l * F_r[k] + srsl * (qr[k] - ql[k])) * iden;
+            out[k] = (S_l >= 0.0f) ? F_l[k] : ((S_r <= 0.0f) ? F_r[k] : star);
+        }
+    }
+}
+
+// CentralUpwindFlux (Euler), EulerCommon.h:171-187
+template <class M>
+__device__ __forceinline__ void euler_cu_flux(const float qm[4], const float qp[4], float gamma, float out[4]) {
+    const float Pp = euler_pressure<M>(qp, gamma);
+    float Fp[4], Fm[4];
+    euler_F<M>(qp, Pp, Fp);
+    const float up = M::div(qp[1], qp[0]);
+    const float cp = M::sqrt(M::div(gamma * Pp, qp[0]));
+    const float Pm = euler_pressure<M>(qm, gamma);
+    euler_F<M>(qm, Pm, Fm);
+    const float um = M::div(qm[1], qm[0]);
+    const float cm = M::sqrt(M::div(gamma * Pm, qm[0]));
+    const float am = fminf(fminf(um - cm, up - cp), 0.0f);
+    const float ap = fmaxf(fmaxf(um + cm, up + cp), 0.0f);
+    const float apam = ap * am;
+    if constexpr (M::exact) {
+        const float den = ap - am;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) out[k] = ((ap * Fm[k] - am * Fp[k]) + apam * (qp[k] - qm[k])) / den;
+    } else {
+        const float iden = M::rcp(ap - am);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) out[k] = ((ap * Fm[k] - am * Fp[k]) + apam * (qp[k] - qm[k])) * iden;
+    }
+}
+
+// writeCfl per-cell value (Euler), EulerCommon.h:41-49: sqrt(gamma*rho) as written
+template <class M>
+__device__ __forceinline__ float euler_cell_dt(const float q[4], float dx, float dy, float gamma) {
+    const float rho = q[0];
+    const float u = M::div(q[1], rho);
+    const float v = M::div(q[2], rho);
+    const float max_u = M::div(dx, fabsf(u) + M::sqrt(gamma * rho));
+    const float max_v = M::div(dy, fabsf(v) + M::sqrt(gamma * rho));
+    return fminf(max_u, max_v);
+}
+
+}  // namespace fvmi

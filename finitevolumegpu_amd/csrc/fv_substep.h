@@ -1,0 +1,483 @@
+// Fused dimensionally split substep: ghost fill + first sweep + second sweep
+// (+ gravity) + store + CFL minimum, one pass over HBM.
+//
+// Replaces the tile kernels KP07DimsplitKernel (GPUSimulators/cuda/SWE2D_KP07_dimsplit.cu:123-205),
+// HLL2Kernel (SWE2D_HLL2.cu:126-214), WAFKernel (SWE2D_WAF.cu:103-176) and the
+// Euler KP07DimsplitKernel (EE2D_KP07_dimsplit.cu:126-247).
+//
+// Work decomposition (gfx950, 64-lane wavefronts).  A wavefront owns a
+// "strip": 60 output columns x `tile_rows` output rows.  Lane c holds padded
+// column x0+c (x0 = 60*strip), so lanes 2..61 are the strip's own columns and
+// lanes 0,1,62,63 its x halo.  The wave marches down the strip one padded row
+// at a time (tile_rows+4 rows including the 2+2 y halo):
+//   * the x sweep of a row is done across lanes, neighbours exchanged with
+//     cross-lane moves, no LDS and no barrier;
+//   * the y sweep is done per lane with a register window of the last 3
+//     (slope schemes) or 4 (WAF) rows, so every y flux is computed once.
+// The reference's "X then Y" (step 0) / "Y then X" (step 1) order, its
+// extended first sweep over the halo lines, and its read/write masks are
+// preserved exactly (GPUSimulators/cuda/common.h:320-475).
+//
+// Ghost cells are not materialised: each load maps its padded index through the
+// boundary rule (periodic wrap, mirror with sign, or the memory value) so the
+// kernel reads the values that readBlock + handleReflectiveBoundary would have
+// produced, in global form (correct for every grid size, unlike the tile code's
+// nx%bw==1 case).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fv_math.h"
+
+namespace fvmi {
+
+enum : int { BC_DIRICHLET = 0, BC_NEUMANN = 1, BC_PERIODIC = 2, BC_REFLECTIVE = 3, BC_OUTFLOW = 4 };
+
+constexpr int kLanes = 64;
+constexpr int kStripCols = 60;  // output columns per wavefront
+
+struct SubstepArgs {
+    int nx, ny;
+    float dx, dy, dt, g, theta, gamma;
+    int step, bc;
+    // interior sub-area [x0,x1) x [y0,y1) to update (0,0,nx,ny = whole domain)
+    int x0, y0, x1, y1;
+    const float* in[4];
+    int in_pitch[4];  // bytes
+    float* out[4];
+    int out_pitch[4];
+    float* cfl;  // two-slot CFL minimum (slot = step&1 written, the other reset); may be null
+    int tile_rows;
+    int strips_x, strips_y;
+    int waves_per_block;
+    int block_cols;  // blocks along x = ceil(strips_x / waves_per_block)
+    int num_blocks;  // logical blocks
+    int xcd_remap;
+};
+
+struct AxisMap {
+    int idx;
+    bool mirrored;
+};
+
+// Memory index read for padded index k along an axis of n interior cells
+// (common.h:247-288 and the mirror rules of :133-239).  Indices past the padded
+// array are clamped as readBlock does (those lanes/rows are never stored).
+__device__ __forceinline__ AxisMap map_axis(int k, int n, int bc_lo, int bc_hi) {
+    if (k < 2) {
+        if (bc_lo == BC_PERIODIC) return {k + n, false};
+        if (bc_lo == BC_REFLECTIVE) return {3 - k, true};
+        if (bc_lo == BC_OUTFLOW) return {2, false};
+        return {k, false};
+    }
+    if (k >= n + 2) {
+        if (k > n + 3) k = n + 3;
+        if (bc_hi == BC_PERIODIC) return {k - n, false};
+        if (bc_hi == BC_REFLECTIVE) return {2 * n + 3 - k, true};
+        if (bc_hi == BC_OUTFLOW) return {n + 1, false};
+        return {k, false};
+    }
+    return {k, false};
+}
+
+// Cross-lane moves: the value held by lane-1 / lane+1, as one DPP row move
+// (wave_shr:1 / wave_shl:1).  Lane 0 (resp. 63) receives 0; those lanes are
+// halo lanes whose results are never stored.
+__device__ __forceinline__ float from_left(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float from_right(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = fminf(v, __shfl_xor(v, off, kLanes));
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Scheme policies
+// ---------------------------------------------------------------------------
+
+template <class M>
+struct SweKP07 {
+    static constexpr int NV = 3;
+    static constexpr bool kSlopes = true;
+    static constexpr bool kEuler = false;
+    __device__ static void pred_F(const float q[3], const SubstepArgs& a, float F[3]) { swe_F<M>(q, a.g, F); }
+    __device__ static void flux_x(const float l[3], const float r[3], const SubstepArgs& a, float o[3]) {
+        swe_cu_flux<M>(l, r, a.g, o);
+    }
+    __device__ static void flux_y(const float l[3], const float r[3], const SubstepArgs& a, float o[3]) {
+        swe_cu_flux<M>(l, r, a.g, o);
+    }
+    __device__ static float cell_dt(const float q[3], const SubstepArgs& a) { return swe_cell_dt<M>(q, a.dx, a.dy, a.g); }
+};
+
+template <class M>
+struct SweHLL2 {
+    static constexpr int NV = 3;
+    static constexpr bool kSlopes = true;
+    static constexpr bool kEuler = false;
+    __device__ static void pred_F(const float q[3], const SubstepArgs& a, float F[3]) { swe_F<M>(q, a.g, F); }
+    __device__ static void flux_x(const float l[3], const float r[3], const SubstepArgs& a, float o[3]) {
+        swe_hll_flux<M>(l, r, a.g, o);
+    }
+    __device__ static void flux_y(const float l[3], const float r[3], const SubstepArgs& a, float o[3]) {
+        swe_hll_flux<M>(l, r, a.g, o);
+    }
+    __device__ static float cell_dt(const float q[3], const SubstepArgs& a) { return swe_cell_dt<M>(q, a.dx, a.dy, a.g); }
+};
+
+template <class M>
+struct SweWAF {
+    static constexpr int NV = 3;
+    static constexpr bool kSlopes = false;
+    static constexpr bool kEuler = false;
+    __device__ static float cell_dt(const float q[3], const SubstepArgs& a) { return swe_cell_dt<M>(q, a.dx, a.dy, a.g); }
+};
+
+template <class M>
+struct EulerKP07 {
+    static constexpr int NV = 4;
+    static constexpr bool kSlopes = true;
+    static constexpr bool kEuler = true;
+    // The predictor passes gamma in the pressure slot: EE2D_KP07_dimsplit.cu:57-58, 102-103.
+    __device__ static void pred_F(const float q[4], const SubstepArgs& a, float F[4]) { euler_F<M>(q, a.gamma, F); }
+    __device__ static void flux_x(const float l[4], const float r[4], const SubstepArgs& a, float o[4]) {
+        euler_hll_flux<M>(l, r, a.gamma, o);
+    }
+    __device__ static void flux_y(const float l[4], const float r[4], const SubstepArgs& a, float o[4]) {
+        euler_cu_flux<M>(l, r, a.gamma, o);
+    }
+    __device__ static float cell_dt(const float q[4], const SubstepArgs& a) {
+        return euler_cell_dt<M>(q, a.dx, a.dy, a.gamma);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Sweep pieces
+// ---------------------------------------------------------------------------
+
+// y-sweep order: (rho, normal = y momentum, tangential = x momentum[, E])
+template <int NV>
+__device__ __forceinline__ void to_y(const float q[NV], float o[NV]) {
+    o[0] = q[0];
+    o[1] = q[2];
+    o[2] = q[1];
+    if constexpr (NV == 4) o[3] = q[3];
+}
+
+// Reconstruction + half-step predictor of one cell (computeFluxF, SWE2D_KP07_dimsplit.cu:42-58):
+// slope from (l, c, r); returns the predicted left-face (qbl) and right-face (qbr) states.
+template <class S, class M>
+__device__ __forceinline__ void reconstruct(const float l[S::NV], const float c[S::NV], const float r[S::NV],
+                                            const SubstepArgs& a, float kk, float qbl[S::NV], float qbr[S::NV]) {
+    constexpr int NV = S::NV;
+    float QL[NV], QR[NV], FL[NV], FR[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const float s = minmod_slope<M>(l[k], c[k], r[k], a.theta);
+        QL[k] = c[k] - 0.5f * s;
+        QR[k] = c[k] + 0.5f * s;
+    }
+    S::pred_F(QL, a, FL);
+    S::pred_F(QR, a, FR);
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        qbl[k] = QL[k] + kk * (FL[k] - FR[k]);
+        qbr[k] = QR[k] + kk * (FL[k] - FR[k]);
+    }
+}
+
+template <class M>
+__device__ __forceinline__ float evolve(float q, float f_in, float f_out, float dt, float d) {
+    if constexpr (M::exact) {
+        return q + (f_in - f_out) * dt / d;  // evolveF/evolveG, common.h:450, 471
+    } else {
+        return q + (f_in - f_out) * (dt * M::rcp(d));
+    }
+}
+
+// x sweep of one padded row held across the 64 lanes (valid on lanes 2..61).
+template <class S, class M>
+__device__ __forceinline__ void x_sweep_row(float q[S::NV], const SubstepArgs& a) {
+    constexpr int NV = S::NV;
+    float F[NV];
+    if constexpr (S::kSlopes) {
+        float l[NV], r[NV], qbl[NV], qbr[NV], qbl_r[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            l[k] = from_left(q[k]);
+            r[k] = from_right(q[k]);
+        }
+        const float kk = M::exact ? a.dt / (2.0f * a.dx) : a.dt * M::rcp(2.0f * a.dx);
+        reconstruct<S, M>(l, q, r, a, kk, qbl, qbr);
+#pragma unroll
+        for (int k = 0; k < NV; ++k) qbl_r[k] = from_right(qbl[k]);
+        S::flux_x(qbr, qbl_r, a, F);  // interface (c, c+1)
+    } else {
+        float l[3], r[3], rr[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            l[k] = from_left(q[k]);
+            r[k] = from_right(q[k]);
+            rr[k] = from_right(r[k]);
+        }
+        swe_waf_flux<M>(l, q, r, rr, a.g, a.dx, a.dt, F);
+    }
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const float Fl = from_left(F[k]);
+        q[k] = evolve<M>(q[k], Fl, F[k], a.dt, a.dx);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The kernel
+// ---------------------------------------------------------------------------
+
+// Per-wave constants of a strip.
+struct Strip {
+    int lane;
+    int col;       // padded column of this lane
+    int row0;      // padded row of r = 0
+    int r_end;     // rows r in [0, r_end)
+    int lane_off;  // byte offset of this lane's column in a row (after the boundary map)
+    bool mirror_x;
+    bool lane_out;  // lane writes its column
+    int bcN, bcS;
+};
+
+template <int NV>
+__device__ __forceinline__ void load_row(const SubstepArgs& a, const Strip& st, int r, float v[NV]) {
+    // The row index is wave-uniform: the row base lives in scalar registers and
+    // each lane adds its 32-bit column offset.
+    const AxisMap my = map_axis(st.row0 + r, a.ny, st.bcS, st.bcN);
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const char* rowp = reinterpret_cast<const char*>(a.in[k]) + (size_t)my.idx * (size_t)a.in_pitch[k];
+        const float x = *reinterpret_cast<const float*>(rowp + st.lane_off);
+        // x mirrors flip the x momentum, y mirrors the y momentum
+        const bool flip = (k == 1 && st.mirror_x) || (k == 2 && my.mirrored);
+        v[k] = flip ? -x : x;
+    }
+}
+
+// Loop-carried state of the y sweep for slope schemes: the predicted top face
+// of the previous row and the flux through its top interface.
+template <class S, class M, bool XFirst>
+__device__ __forceinline__ void slope_row(const SubstepArgs& a, const Strip& st, float kky,
+                                          const float (&w0)[S::NV], const float (&w1)[S::NV], float (&w2)[S::NV],
+                                          const float (&qbt_in)[S::NV], float (&qbt_out)[S::NV],
+                                          const float (&g_in)[S::NV], float (&g_out)[S::NV], int r, bool emit,
+                                          float& dt_min) {
+    // w0, w1: rows r-2, r-1 (already x swept when XFirst); w2: raw row r on entry.
+    constexpr int NV = S::NV;
+    if constexpr (XFirst) x_sweep_row<S, M>(w2, a);
+    float y0[NV], y1[NV], y2[NV], qbb[NV];
+    to_y<NV>(w0, y0);
+    to_y<NV>(w1, y1);
+    to_y<NV>(w2, y2);
+    reconstruct<S, M>(y0, y1, y2, a, kky, qbb, qbt_out);  // faces of row r-1
+    float G[NV];
+    S::flux_y(qbt_in, qbb, a, G);  // interface (r-2, r-1)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) g_out[k] = G[k];
+    if (!emit) return;
+    float outq[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) y0[k] = evolve<M>(y0[k], g_in[k], G[k], a.dt, a.dy);
+    to_y<NV>(y0, outq);  // row r-2 after the y sweep
+    if constexpr (!XFirst) x_sweep_row<S, M>(outq, a);
+    if constexpr (S::kEuler) {
+        // gravity source, EE2D_KP07_dimsplit.cu:197-205
+        if (a.g > 0.0f) {
+            const float rho_v = outq[2];
+            outq[2] -= a.g * outq[0] * a.dt;
+            outq[3] -= a.g * rho_v * a.dt;
+        }
+    }
+    const int prow = st.row0 + r - 2;
+    if (st.lane_out) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+            *reinterpret_cast<float*>(reinterpret_cast<char*>(a.out[k]) + (size_t)prow * (size_t)a.out_pitch[k] +
+                                      (size_t)st.col * sizeof(float)) = outq[k];
+    }
+    const float cd = S::cell_dt(outq, a);
+    dt_min = st.lane_out ? fminf(dt_min, cd) : dt_min;
+}
+
+// WAF: the y window holds 4 rows and there is no slope state.
+template <class S, class M, bool XFirst>
+__device__ __forceinline__ void waf_row(const SubstepArgs& a, const Strip& st, const float (&w0)[3],
+                                        const float (&w1)[3], const float (&w2)[3], float (&w3)[3],
+                                        const float (&g_in)[3], float (&g_out)[3], int r, bool emit, float& dt_min) {
+    if constexpr (XFirst) x_sweep_row<S, M>(w3, a);
+    float y0[3], y1[3], y2[3], y3[3], G[3];
+    to_y<3>(w0, y0);
+    to_y<3>(w1, y1);
+    to_y<3>(w2, y2);
+    to_y<3>(w3, y3);
+    swe_waf_flux<M>(y0, y1, y2, y3, a.g, a.dy, a.dt, G);  // interface (r-2, r-1)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) g_out[k] = G[k];
+    if (!emit) return;
+    float outq[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) y1[k] = evolve<M>(y1[k], g_in[k], G[k], a.dt, a.dy);
+    to_y<3>(y1, outq);
+    if constexpr (!XFirst) x_sweep_row<S, M>(outq, a);
+    const int prow = st.row0 + r - 2;
+    if (st.lane_out) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            *reinterpret_cast<float*>(reinterpret_cast<char*>(a.out[k]) + (size_t)prow * (size_t)a.out_pitch[k] +
+                                      (size_t)st.col * sizeof(float)) = outq[k];
+    }
+    const float cd = S::cell_dt(outq, a);
+    dt_min = st.lane_out ? fminf(dt_min, cd) : dt_min;
+}
+
+template <int NV>
+__device__ __forceinline__ void copyv(float (&d)[NV], const float (&s)[NV]) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) d[k] = s[k];
+}
+
+template <class S, class M, bool XFirst>
+__device__ __forceinline__ float march_slopes(const SubstepArgs& a, const Strip& st) {
+    constexpr int NV = S::NV;
+    const float kky = M::exact ? a.dt / (2.0f * a.dy) : a.dt * M::rcp(2.0f * a.dy);
+    float dt_min = __builtin_inff();
+    float A[NV], B[NV], C[NV], P0[NV], P1[NV], G0[NV], G1[NV];
+    const int last = st.r_end - 1;
+    // prologue: rows 0, 1 (window fill), row 2 (first faces), row 3 (first flux)
+    load_row<NV>(a, st, 0, A);
+    load_row<NV>(a, st, 1, B);
+    if constexpr (XFirst) {
+        x_sweep_row<S, M>(A, a);
+        x_sweep_row<S, M>(B, a);
+    }
+    load_row<NV>(a, st, 2, C);
+    {
+        // row 2: faces of row 1 only (no flux yet)
+        if constexpr (XFirst) x_sweep_row<S, M>(C, a);
+        float y0[NV], y1[NV], y2[NV], qbb[NV];
+        to_y<NV>(A, y0);
+        to_y<NV>(B, y1);
+        to_y<NV>(C, y2);
+        reconstruct<S, M>(y0, y1, y2, a, kky, qbb, P0);
+    }
+    // row 3: flux (1,2), no output.  Window becomes (B, C, A).
+    load_row<NV>(a, st, 3, A);
+    slope_row<S, M, XFirst>(a, st, kky, B, C, A, P0, P1, G1, G0, 3, false, dt_min);
+    // now rows (C, A) = (2, 3) hold x-swept rows 2, 3; P1 = top face of row 2; G0 = flux (1,2)
+    int r = 4;
+    float N0[NV], N1[NV];
+    load_row<NV>(a, st, min(r, last), N0);
+    for (; r + 6 <= st.r_end; r += 6) {
+        load_row<NV>(a, st, min(r + 1, last), N1);
+        copyv<NV>(B, N0);
+        slope_row<S, M, XFirst>(a, st, kky, C, A, B, P1, P0, G0, G1, r, true, dt_min);
+        load_row<NV>(a, st, min(r + 2, last), N0);
+        copyv<NV>(C, N1);
+        slope_row<S, M, XFirst>(a, st, kky, A, B, C, P0, P1, G1, G0, r + 1, true, dt_min);
+        load_row<NV>(a, st, min(r + 3, last), N1);
+        copyv<NV>(A, N0);
+        slope_row<S, M, XFirst>(a, st, kky, B, C, A, P1, P0, G0, G1, r + 2, true, dt_min);
+        load_row<NV>(a, st, min(r + 4, last), N0);
+        copyv<NV>(B, N1);
+        slope_row<S, M, XFirst>(a, st, kky, C, A, B, P0, P1, G1, G0, r + 3, true, dt_min);
+        load_row<NV>(a, st, min(r + 5, last), N1);
+        copyv<NV>(C, N0);
+        slope_row<S, M, XFirst>(a, st, kky, A, B, C, P1, P0, G0, G1, r + 4, true, dt_min);
+        load_row<NV>(a, st, min(r + 6, last), N0);
+        copyv<NV>(A, N1);
+        slope_row<S, M, XFirst>(a, st, kky, B, C, A, P0, P1, G1, G0, r + 5, true, dt_min);
+    }
+    // remainder: window (C, A), P1, G0 as at the loop head
+    for (; r < st.r_end; ++r) {
+        copyv<NV>(B, N0);
+        if (r + 1 < st.r_end) load_row<NV>(a, st, r + 1, N0);
+        slope_row<S, M, XFirst>(a, st, kky, C, A, B, P1, P0, G0, G1, r, true, dt_min);
+        copyv<NV>(C, A);
+        copyv<NV>(A, B);
+        copyv<NV>(P1, P0);
+        copyv<NV>(G0, G1);
+    }
+    return dt_min;
+}
+
+template <class S, class M, bool XFirst>
+__device__ __forceinline__ float march_waf(const SubstepArgs& a, const Strip& st) {
+    float dt_min = __builtin_inff();
+    float A[3], B[3], C[3], D[3], G0[3], G1[3];
+    load_row<3>(a, st, 0, A);
+    load_row<3>(a, st, 1, B);
+    load_row<3>(a, st, 2, C);
+    if constexpr (XFirst) {
+        x_sweep_row<S, M>(A, a);
+        x_sweep_row<S, M>(B, a);
+        x_sweep_row<S, M>(C, a);
+    }
+    load_row<3>(a, st, 3, D);
+    waf_row<S, M, XFirst>(a, st, A, B, C, D, G1, G0, 3, false, dt_min);  // flux (1,2)
+    // window (B, C, D) = rows 1..3, G0 = flux (1,2)
+    for (int r = 4; r < st.r_end; ++r) {
+        load_row<3>(a, st, r, A);
+        waf_row<S, M, XFirst>(a, st, B, C, D, A, G0, G1, r, true, dt_min);
+        copyv<3>(B, C);
+        copyv<3>(C, D);
+        copyv<3>(D, A);
+        copyv<3>(G0, G1);
+    }
+    return dt_min;
+}
+
+template <class S, class M>
+__global__ void __launch_bounds__(256) substep_kernel(SubstepArgs a) {
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int wave = threadIdx.x >> 6;
+    if (a.cfl != nullptr && blockIdx.x == 0 && threadIdx.x == 0) a.cfl[1 - (a.step & 1)] = __builtin_inff();
+    // Hardware dispatch places workgroup i on XCD i % 8; remap so that each XCD
+    // owns a contiguous range of logical blocks (neighbouring strips share L2).
+    int lb = blockIdx.x;
+    if (a.xcd_remap) {
+        const int per = gridDim.x >> 3;  // gridDim.x is a multiple of 8 when remapping
+        lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    }
+    if (lb >= a.num_blocks) return;
+    const int strip_x = (lb % a.block_cols) * a.waves_per_block + wave;
+    const int strip_y = lb / a.block_cols;
+    if (strip_x >= a.strips_x) return;
+
+    Strip st;
+    st.lane = lane;
+    st.col = a.x0 + kStripCols * strip_x + lane;
+    st.row0 = a.y0 + a.tile_rows * strip_y;
+    st.r_end = min(a.tile_rows + 4, a.y1 + 4 - st.row0);  // never past padded row y1+3 <= ny+3
+    st.bcN = (a.bc >> 24) & 0xF;
+    st.bcS = (a.bc >> 16) & 0xF;
+    const AxisMap mx = map_axis(st.col, a.nx, a.bc & 0xF, (a.bc >> 8) & 0xF);
+    st.lane_off = mx.idx * (int)sizeof(float);
+    st.mirror_x = mx.mirrored;
+    st.lane_out = lane >= 2 && lane < 2 + kStripCols && st.col < a.x1 + 2;
+
+    float dt_min;
+    if constexpr (S::kSlopes) {
+        dt_min = (a.step == 0) ? march_slopes<S, M, true>(a, st) : march_slopes<S, M, false>(a, st);
+    } else {
+        dt_min = (a.step == 0) ? march_waf<S, M, true>(a, st) : march_waf<S, M, false>(a, st);
+    }
+    if (a.cfl != nullptr) {
+        const float m = wave_min(dt_min);
+        if (lane == 0 && m < __builtin_inff())
+            atomicMin(reinterpret_cast<unsigned int*>(a.cfl + (a.step & 1)), __float_as_uint(m));
+    }
+}
+
+}  // namespace fvmi

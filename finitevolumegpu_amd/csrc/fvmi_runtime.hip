@@ -1,0 +1,426 @@
+// C ABI of libfvmi.so: device runtime, memory, and the validated entry points of
+// the substep kernels.  Declarations and reference citations: include/fvmi.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "../../include/fvmi.h"
+#include "fv_launch.h"
+
+using namespace fvmi;
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    return fail(FVMI_ERR_HIP, "%s: %s (%d)", what, hipGetErrorString(e), (int)e);
+}
+
+#define FVMI_HIP(call)                                   \
+    do {                                                 \
+        hipError_t e_ = (call);                          \
+        if (e_ != hipSuccess) return hip_fail(e_, #call); \
+    } while (0)
+
+struct Ctx {
+    int device;
+};
+
+hipMemcpyKind kind_of(int kind) {
+    switch (kind) {
+        case FVMI_MEMCPY_HTOD: return hipMemcpyHostToDevice;
+        case FVMI_MEMCPY_DTOH: return hipMemcpyDeviceToHost;
+        case FVMI_MEMCPY_DTOD: return hipMemcpyDeviceToDevice;
+        default: return hipMemcpyDefault;
+    }
+}
+
+bool valid_bc(int side) { return side == BC_DIRICHLET || side == BC_PERIODIC || side == BC_REFLECTIVE || side == BC_OUTFLOW; }
+
+int check_bc(int bc) {
+    const int sides[4] = {(bc >> 24) & 0xF, (bc >> 16) & 0xF, (bc >> 8) & 0xF, bc & 0xF};
+    for (int s : sides) {
+        if (s == BC_NEUMANN) return fail(FVMI_ERR_UNSUPPORTED, "Neumann boundary condition not supported");
+        if (!valid_bc(s)) return fail(FVMI_ERR_INVALID, "invalid boundary condition code %d", s);
+    }
+    return FVMI_OK;
+}
+
+int fill_launch(const fvmi_launch_t* l, SubstepArgs& a, int* math) {
+    a.tile_rows = (l && l->tile_rows > 0) ? l->tile_rows : 64;
+    a.waves_per_block = (l && l->waves_per_block > 0) ? l->waves_per_block : 4;
+    a.xcd_remap = (l && l->xcd_remap == 2) ? 0 : 1;
+    *math = l ? l->math : FVMI_MATH_FAST;
+    if (a.tile_rows > 4096) return fail(FVMI_ERR_INVALID, "tile_rows %d out of range [1, 4096]", a.tile_rows);
+    if (a.waves_per_block > 4) return fail(FVMI_ERR_INVALID, "waves_per_block %d out of range [1, 4]", a.waves_per_block);
+    if (*math != FVMI_MATH_FAST && *math != FVMI_MATH_EXACT) return fail(FVMI_ERR_INVALID, "unknown math mode %d", *math);
+    return FVMI_OK;
+}
+
+int run_substep(int scheme, SubstepArgs& a, int nvars, const fvmi_launch_t* launch, void* stream) {
+    if (a.nx < 2 || a.ny < 2) return fail(FVMI_ERR_INVALID, "grid %dx%d too small (need nx, ny >= 2)", a.nx, a.ny);
+    if ((size_t)(a.ny + 4) * (size_t)(a.nx + 4) > (size_t)1 << 34)
+        return fail(FVMI_ERR_INVALID, "grid %dx%d too large", a.nx, a.ny);
+    if (a.step != 0 && a.step != 1) return fail(FVMI_ERR_INVALID, "step must be 0 or 1, got %d", a.step);
+    if (int rc = check_bc(a.bc)) return rc;
+    if (!(a.dt >= 0.0f) || !std::isfinite(a.dt)) return fail(FVMI_ERR_INVALID, "dt must be finite and >= 0");
+    if (!(a.dx > 0.0f) || !(a.dy > 0.0f)) return fail(FVMI_ERR_INVALID, "dx, dy must be > 0");
+    const int min_pitch = (a.nx + 4) * (int)sizeof(float);
+    for (int k = 0; k < nvars; ++k) {
+        if (!a.in[k] || !a.out[k]) return fail(FVMI_ERR_INVALID, "field pointer %d is NULL", k);
+        if (a.in[k] == a.out[k]) return fail(FVMI_ERR_INVALID, "field %d: input and output alias", k);
+        if (a.in_pitch[k] < min_pitch || a.out_pitch[k] < min_pitch || (a.in_pitch[k] & 3) || (a.out_pitch[k] & 3))
+            return fail(FVMI_ERR_INVALID, "field %d: pitch must be >= %d bytes and a multiple of 4", k, min_pitch);
+    }
+    if (a.x1 == 0) a.x1 = a.nx;
+    if (a.y1 == 0) a.y1 = a.ny;
+    if (a.x0 < 0 || a.y0 < 0 || a.x1 > a.nx || a.y1 > a.ny || a.x0 >= a.x1 || a.y0 >= a.y1)
+        return fail(FVMI_ERR_INVALID, "sub-area [%d,%d)x[%d,%d) outside the %dx%d domain", a.x0, a.x1, a.y0, a.y1,
+                    a.nx, a.ny);
+    int math = 0;
+    if (int rc = fill_launch(launch, a, &math)) return rc;
+    hipError_t e = (math == FVMI_MATH_EXACT) ? launch_substep_exact(scheme, a, (hipStream_t)stream)
+                                             : launch_substep_fast(scheme, a, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "substep launch");
+    return FVMI_OK;
+}
+
+__global__ void fill_kernel(float* p, size_t pitch, int w, int h, float v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i < w && j < h) reinterpret_cast<float*>(reinterpret_cast<char*>(p) + (size_t)j * pitch)[i] = v;
+}
+
+__global__ void copy_kernel(float4* __restrict__ dst, const float4* __restrict__ src, size_t n4) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+struct HaloArgs {
+    float* f[4];
+    int pitch[4];
+    int nx, ny, nvars;
+};
+
+// thread = (row, var, one of the 4 ghost columns)
+__global__ void halo_periodic_x_kernel(HaloArgs a) {
+    const int row = blockIdx.x * blockDim.y + threadIdx.y;
+    const int c = threadIdx.x & 3;
+    const int var = threadIdx.x >> 2;
+    if (row >= a.ny + 4 || var >= a.nvars) return;
+    float* r = reinterpret_cast<float*>(reinterpret_cast<char*>(a.f[var]) + (size_t)row * a.pitch[var]);
+    // destination columns 0,1 <- nx, nx+1 ; nx+2, nx+3 <- 2, 3
+    const int dst = (c < 2) ? c : a.nx + c;
+    const int src = (c < 2) ? a.nx + c : c;
+    r[dst] = r[src];
+}
+
+__global__ void field_stats_kernel(int nx, int ny, const float* f, int pitch, double* sum_out, unsigned int* bad_out) {
+    double s = 0.0;
+    unsigned int bad = 0;
+    for (int j = blockIdx.y; j < ny; j += gridDim.y) {
+        const float* row = reinterpret_cast<const float*>(reinterpret_cast<const char*>(f) + (size_t)(j + 2) * pitch);
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nx; i += gridDim.x * blockDim.x) {
+            const float v = row[i + 2];
+            if (std::isfinite(v))
+                s += (double)v;
+            else
+                ++bad;
+        }
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        s += __shfl_xor(s, off, 64);
+        bad += __shfl_xor(bad, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(sum_out, s);
+        if (bad) atomicAdd(bad_out, bad);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fvmi_last_error(void) { return g_err; }
+
+int fvmi_version(void) { return 1; }
+
+int fvmi_device_count(int* count) {
+    if (!count) return fail(FVMI_ERR_INVALID, "count is NULL");
+    FVMI_HIP(hipGetDeviceCount(count));
+    return FVMI_OK;
+}
+
+int fvmi_ctx_create(int device, int flags, void** ctx) {
+    (void)flags;
+    if (!ctx) return fail(FVMI_ERR_INVALID, "ctx is NULL");
+    int n = 0;
+    FVMI_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(FVMI_ERR_INVALID, "device %d out of range (%d devices)", device, n);
+    FVMI_HIP(hipSetDevice(device));
+    FVMI_HIP(hipFree(nullptr));  // force runtime initialisation on this device
+    Ctx* c = new Ctx{device};
+    *ctx = c;
+    return FVMI_OK;
+}
+
+int fvmi_ctx_destroy(void* ctx) {
+    delete static_cast<Ctx*>(ctx);
+    return FVMI_OK;
+}
+
+int fvmi_ctx_make_current(void* ctx) {
+    if (!ctx) return fail(FVMI_ERR_INVALID, "ctx is NULL");
+    FVMI_HIP(hipSetDevice(static_cast<Ctx*>(ctx)->device));
+    return FVMI_OK;
+}
+
+int fvmi_ctx_synchronize(void* ctx) {
+    if (ctx) FVMI_HIP(hipSetDevice(static_cast<Ctx*>(ctx)->device));
+    FVMI_HIP(hipDeviceSynchronize());
+    return FVMI_OK;
+}
+
+int fvmi_ctx_device(void* ctx, int* device) {
+    if (!ctx || !device) return fail(FVMI_ERR_INVALID, "NULL argument");
+    *device = static_cast<Ctx*>(ctx)->device;
+    return FVMI_OK;
+}
+
+int fvmi_device_name(int device, char* buf, int len) {
+    if (!buf || len <= 0) return fail(FVMI_ERR_INVALID, "bad buffer");
+    hipDeviceProp_t p;
+    FVMI_HIP(hipGetDeviceProperties(&p, device));
+    std::snprintf(buf, (size_t)len, "%s (%s)", p.name, p.gcnArchName);
+    return FVMI_OK;
+}
+
+int fvmi_mem_get_info(size_t* free_bytes, size_t* total_bytes) {
+    if (!free_bytes || !total_bytes) return fail(FVMI_ERR_INVALID, "NULL argument");
+    FVMI_HIP(hipMemGetInfo(free_bytes, total_bytes));
+    return FVMI_OK;
+}
+
+int fvmi_stream_create(void** stream) {
+    if (!stream) return fail(FVMI_ERR_INVALID, "stream is NULL");
+    hipStream_t s;
+    FVMI_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+    return FVMI_OK;
+}
+
+int fvmi_stream_destroy(void* stream) {
+    if (stream) FVMI_HIP(hipStreamDestroy((hipStream_t)stream));
+    return FVMI_OK;
+}
+
+int fvmi_stream_synchronize(void* stream) {
+    FVMI_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return FVMI_OK;
+}
+
+int fvmi_event_create(void** event) {
+    if (!event) return fail(FVMI_ERR_INVALID, "event is NULL");
+    hipEvent_t e;
+    FVMI_HIP(hipEventCreate(&e));
+    *event = e;
+    return FVMI_OK;
+}
+
+int fvmi_event_destroy(void* event) {
+    if (event) FVMI_HIP(hipEventDestroy((hipEvent_t)event));
+    return FVMI_OK;
+}
+
+int fvmi_event_record(void* event, void* stream) {
+    FVMI_HIP(hipEventRecord((hipEvent_t)event, (hipStream_t)stream));
+    return FVMI_OK;
+}
+
+int fvmi_event_synchronize(void* event) {
+    FVMI_HIP(hipEventSynchronize((hipEvent_t)event));
+    return FVMI_OK;
+}
+
+int fvmi_event_elapsed_ms(void* start, void* end, float* ms) {
+    if (!ms) return fail(FVMI_ERR_INVALID, "ms is NULL");
+    FVMI_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)end));
+    return FVMI_OK;
+}
+
+int fvmi_malloc(void** ptr, size_t bytes) {
+    if (!ptr) return fail(FVMI_ERR_INVALID, "ptr is NULL");
+    FVMI_HIP(hipMalloc(ptr, bytes));
+    return FVMI_OK;
+}
+
+int fvmi_malloc_pitch(void** ptr, size_t* pitch, size_t width_bytes, size_t height) {
+    if (!ptr || !pitch) return fail(FVMI_ERR_INVALID, "NULL argument");
+    const size_t p = (width_bytes + 255) & ~(size_t)255;
+    FVMI_HIP(hipMalloc(ptr, p * height));
+    *pitch = p;
+    return FVMI_OK;
+}
+
+int fvmi_free(void* ptr) {
+    if (ptr) FVMI_HIP(hipFree(ptr));
+    return FVMI_OK;
+}
+
+int fvmi_host_malloc(void** ptr, size_t bytes) {
+    if (!ptr) return fail(FVMI_ERR_INVALID, "ptr is NULL");
+    FVMI_HIP(hipHostMalloc(ptr, bytes, hipHostMallocDefault));
+    return FVMI_OK;
+}
+
+int fvmi_host_free(void* ptr) {
+    if (ptr) FVMI_HIP(hipHostFree(ptr));
+    return FVMI_OK;
+}
+
+int fvmi_memcpy_async(void* dst, const void* src, size_t bytes, int kind, void* stream) {
+    FVMI_HIP(hipMemcpyAsync(dst, src, bytes, kind_of(kind), (hipStream_t)stream));
+    return FVMI_OK;
+}
+
+int fvmi_memcpy2d_async(void* dst, size_t dpitch, const void* src, size_t spitch, size_t width_bytes, size_t height,
+                        int kind, void* stream) {
+    if (width_bytes > dpitch || width_bytes > spitch) return fail(FVMI_ERR_INVALID, "row width exceeds pitch");
+    FVMI_HIP(hipMemcpy2DAsync(dst, dpitch, src, spitch, width_bytes, height, kind_of(kind), (hipStream_t)stream));
+    return FVMI_OK;
+}
+
+int fvmi_fill_f32(float* ptr, size_t pitch_bytes, int width, int height, float value, void* stream) {
+    if (!ptr || width < 0 || height < 0 || pitch_bytes < (size_t)width * sizeof(float))
+        return fail(FVMI_ERR_INVALID, "bad fill region");
+    if (width == 0 || height == 0) return FVMI_OK;
+    hipLaunchKernelGGL(fill_kernel, dim3((width + 255) / 256, height), dim3(256), 0, (hipStream_t)stream, ptr,
+                       pitch_bytes, width, height, value);
+    FVMI_HIP(hipGetLastError());
+    return FVMI_OK;
+}
+
+int fvmi_copy_f32(float* dst, const float* src, size_t count, void* stream) {
+    if (!dst || !src || (count & 3) || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15))
+        return fail(FVMI_ERR_INVALID, "copy needs 16-byte aligned buffers and a multiple of 4 floats");
+    if (count == 0) return FVMI_OK;
+    const size_t n4 = count / 4;
+    size_t blocks = (n4 + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                       reinterpret_cast<float4*>(dst), reinterpret_cast<const float4*>(src), n4);
+    FVMI_HIP(hipGetLastError());
+    return FVMI_OK;
+}
+
+int fvmi_swe_kp07_dimsplit_substep(int nx, int ny, float dx, float dy, float dt, float g, float theta, int step,
+                                   int boundary_conditions, const float* h0, int h0_pitch, const float* hu0,
+                                   int hu0_pitch, const float* hv0, int hv0_pitch, float* h1, int h1_pitch,
+                                   float* hu1, int hu1_pitch, float* hv1, int hv1_pitch, float* cfl,
+                                   const fvmi_launch_t* launch, void* stream) {
+    SubstepArgs a{};
+    a.nx = nx, a.ny = ny, a.dx = dx, a.dy = dy, a.dt = dt, a.g = g, a.theta = theta, a.step = step;
+    a.bc = boundary_conditions;
+    a.in[0] = h0, a.in[1] = hu0, a.in[2] = hv0;
+    a.in_pitch[0] = h0_pitch, a.in_pitch[1] = hu0_pitch, a.in_pitch[2] = hv0_pitch;
+    a.out[0] = h1, a.out[1] = hu1, a.out[2] = hv1;
+    a.out_pitch[0] = h1_pitch, a.out_pitch[1] = hu1_pitch, a.out_pitch[2] = hv1_pitch;
+    a.cfl = cfl;
+    return run_substep(SCHEME_KP07, a, 3, launch, stream);
+}
+
+int fvmi_swe_hll2_substep(int nx, int ny, float dx, float dy, float dt, float g, float theta, int step,
+                          int boundary_conditions, const float* h0, int h0_pitch, const float* hu0, int hu0_pitch,
+                          const float* hv0, int hv0_pitch, float* h1, int h1_pitch, float* hu1, int hu1_pitch,
+                          float* hv1, int hv1_pitch, float* cfl, const fvmi_launch_t* launch, void* stream) {
+    SubstepArgs a{};
+    a.nx = nx, a.ny = ny, a.dx = dx, a.dy = dy, a.dt = dt, a.g = g, a.theta = theta, a.step = step;
+    a.bc = boundary_conditions;
+    a.in[0] = h0, a.in[1] = hu0, a.in[2] = hv0;
+    a.in_pitch[0] = h0_pitch, a.in_pitch[1] = hu0_pitch, a.in_pitch[2] = hv0_pitch;
+    a.out[0] = h1, a.out[1] = hu1, a.out[2] = hv1;
+    a.out_pitch[0] = h1_pitch, a.out_pitch[1] = hu1_pitch, a.out_pitch[2] = hv1_pitch;
+    a.cfl = cfl;
+    return run_substep(SCHEME_HLL2, a, 3, launch, stream);
+}
+
+int fvmi_swe_waf_substep(int nx, int ny, float dx, float dy, float dt, float g, int step, int boundary_conditions,
+                         const float* h0, int h0_pitch, const float* hu0, int hu0_pitch, const float* hv0,
+                         int hv0_pitch, float* h1, int h1_pitch, float* hu1, int hu1_pitch, float* hv1,
+                         int hv1_pitch, float* cfl, const fvmi_launch_t* launch, void* stream) {
+    SubstepArgs a{};
+    a.nx = nx, a.ny = ny, a.dx = dx, a.dy = dy, a.dt = dt, a.g = g, a.step = step;
+    a.bc = boundary_conditions;
+    a.in[0] = h0, a.in[1] = hu0, a.in[2] = hv0;
+    a.in_pitch[0] = h0_pitch, a.in_pitch[1] = hu0_pitch, a.in_pitch[2] = hv0_pitch;
+    a.out[0] = h1, a.out[1] = hu1, a.out[2] = hv1;
+    a.out_pitch[0] = h1_pitch, a.out_pitch[1] = hu1_pitch, a.out_pitch[2] = hv1_pitch;
+    a.cfl = cfl;
+    return run_substep(SCHEME_WAF, a, 3, launch, stream);
+}
+
+int fvmi_euler_kp07_dimsplit_substep(int nx, int ny, float dx, float dy, float dt, float g, float gamma, float theta,
+                                     int step, int boundary_conditions, const float* rho0, int rho0_pitch,
+                                     const float* rho_u0, int rho_u0_pitch, const float* rho_v0, int rho_v0_pitch,
+                                     const float* E0, int E0_pitch, float* rho1, int rho1_pitch, float* rho_u1,
+                                     int rho_u1_pitch, float* rho_v1, int rho_v1_pitch, float* E1, int E1_pitch,
+                                     float* cfl, int x0, int y0, int x1, int y1, const fvmi_launch_t* launch,
+                                     void* stream) {
+    SubstepArgs a{};
+    a.nx = nx, a.ny = ny, a.dx = dx, a.dy = dy, a.dt = dt, a.g = g, a.gamma = gamma, a.theta = theta;
+    a.step = step, a.bc = boundary_conditions;
+    a.in[0] = rho0, a.in[1] = rho_u0, a.in[2] = rho_v0, a.in[3] = E0;
+    a.in_pitch[0] = rho0_pitch, a.in_pitch[1] = rho_u0_pitch, a.in_pitch[2] = rho_v0_pitch, a.in_pitch[3] = E0_pitch;
+    a.out[0] = rho1, a.out[1] = rho_u1, a.out[2] = rho_v1, a.out[3] = E1;
+    a.out_pitch[0] = rho1_pitch, a.out_pitch[1] = rho_u1_pitch, a.out_pitch[2] = rho_v1_pitch;
+    a.out_pitch[3] = E1_pitch;
+    a.cfl = cfl;
+    a.x0 = x0, a.y0 = y0, a.x1 = x1, a.y1 = y1;
+    return run_substep(SCHEME_EULER, a, 4, launch, stream);
+}
+
+int fvmi_halo_copy_periodic_x(int nx, int ny, int nvars, float* const* fields, const int* pitches, void* stream) {
+    if (nx < 2 || ny < 0 || nvars < 1 || nvars > 4 || !fields || !pitches)
+        return fail(FVMI_ERR_INVALID, "bad halo copy arguments");
+    HaloArgs a{};
+    a.nx = nx, a.ny = ny, a.nvars = nvars;
+    for (int k = 0; k < nvars; ++k) {
+        if (!fields[k] || pitches[k] < (nx + 4) * (int)sizeof(float))
+            return fail(FVMI_ERR_INVALID, "bad field %d for halo copy", k);
+        a.f[k] = fields[k];
+        a.pitch[k] = pitches[k];
+    }
+    const int rows_per_block = 16;
+    const dim3 block(16, rows_per_block);  // x: 4 vars x 4 ghost columns
+    const dim3 grid((ny + 4 + rows_per_block - 1) / rows_per_block);
+    hipLaunchKernelGGL(halo_periodic_x_kernel, grid, block, 0, (hipStream_t)stream, a);
+    FVMI_HIP(hipGetLastError());
+    return FVMI_OK;
+}
+
+int fvmi_field_stats(int nx, int ny, const float* field, int pitch, double* sum_out, unsigned int* nonfinite_out,
+                     void* stream) {
+    if (nx < 1 || ny < 1 || !field || !sum_out || !nonfinite_out || pitch < (nx + 4) * (int)sizeof(float))
+        return fail(FVMI_ERR_INVALID, "bad field_stats arguments");
+    const int bx = (nx + 255) / 256 < 16 ? (nx + 255) / 256 : 16;
+    const int by = ny < 4096 ? ny : 4096;
+    hipLaunchKernelGGL(field_stats_kernel, dim3(bx, by), dim3(256), 0, (hipStream_t)stream, nx, ny, field, pitch,
+                       sum_out, nonfinite_out);
+    FVMI_HIP(hipGetLastError());
+    return FVMI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,165 @@
+/*
+ * fvmi.h -- C ABI of libfvmi.so, the MI355X (gfx950) flux-and-update engine.
+ *
+ * This library replaces, for the dimensionally split hot path of
+ * FiniteVolumeGPU, the PyCUDA device runtime (GPUSimulators/CudaContext.py:45-272),
+ * the pitched padded device arrays (GPUSimulators/Common.py:498-616, 741-779)
+ * and the CUDA kernels launched through `prepared_async_call`
+ * (GPUSimulators/KP07_dimsplit.py:111-126, HLL2.py, WAF.py, EE2D_KP07_dimsplit.py:115-266).
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative code on failure;
+ *     fvmi_last_error() then returns a message (thread local).  No C++
+ *     exception crosses this boundary.
+ *   - Streams and events are opaque `void*` (hipStream_t / hipEvent_t).
+ *     A NULL stream is the device's null stream.
+ *   - Field buffers are row-major float32 of (ny+4) rows x pitch bytes with
+ *     the interior at rows 2..ny+1, columns 2..nx+1 (2 ghost cells per side),
+ *     the layout of CudaArray2D (Common.py:502-531).  Pitches are in bytes, as
+ *     the reference passes `strides[0]`.
+ *   - Boundary conditions are packed N<<24 | S<<16 | E<<8 | W with
+ *     0 = Dirichlet, 2 = Periodic, 3 = Reflective (Simulator.py:44-92,
+ *     common.h:109-130) and 4 = Outflow (zero-gradient ghost copy; an
+ *     extension with no counterpart in the reference).  Neumann (1) is
+ *     rejected, as Simulator.py:68-72 does.
+ */
+#ifndef FVMI_H
+#define FVMI_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FVMI_OK 0
+#define FVMI_ERR_INVALID (-1)
+#define FVMI_ERR_HIP (-2)
+#define FVMI_ERR_UNSUPPORTED (-3)
+
+#define FVMI_MATH_FAST 0  /* hardware rcp/sqrt, contraction: judged at rel-L1 <= 1e-5 */
+#define FVMI_MATH_EXACT 1 /* IEEE div/sqrt, no FMA: bitwise equal to an IEEE float32 evaluation */
+
+#define FVMI_MEMCPY_HTOD 1
+#define FVMI_MEMCPY_DTOH 2
+#define FVMI_MEMCPY_DTOD 3
+
+/* Launch shape of the substep kernels.  A NULL pointer, or a zero field, selects
+ * the default.  Replaces the compile-time BLOCK_WIDTH/BLOCK_HEIGHT defines of
+ * the reference kernels (KP07_dimsplit.py:80-89). */
+typedef struct fvmi_launch {
+    int tile_rows;       /* output rows per 64-lane column strip (default 64) */
+    int waves_per_block; /* strips per workgroup, 1..4 (default 4) */
+    int math;            /* FVMI_MATH_FAST (default) or FVMI_MATH_EXACT */
+    int xcd_remap;       /* 1 (default): consecutive strips on one XCD; 2: off */
+} fvmi_launch_t;
+
+/* ---- errors, version ------------------------------------------------------ */
+const char* fvmi_last_error(void);
+int fvmi_version(void);
+
+/* ---- device / context: CudaContext.__init__, synchronize (CudaContext.py:45-120, 262-263) */
+int fvmi_device_count(int* count);
+/* Selects `device` for the calling thread and initialises it; `flags` is reserved (0). */
+int fvmi_ctx_create(int device, int flags, void** ctx);
+int fvmi_ctx_destroy(void* ctx);
+/* Makes ctx's device current for the calling thread (CudaContext's context push). */
+int fvmi_ctx_make_current(void* ctx);
+int fvmi_ctx_synchronize(void* ctx);
+int fvmi_ctx_device(void* ctx, int* device);
+int fvmi_device_name(int device, char* buf, int len);
+/* cuda.mem_get_info(), CudaContext.py:81 */
+int fvmi_mem_get_info(size_t* free_bytes, size_t* total_bytes);
+
+/* ---- streams and events: cuda.Stream (Simulator.py:164-165), cuda.Event (Autotuner.py:186-205) */
+int fvmi_stream_create(void** stream);
+int fvmi_stream_destroy(void* stream);
+int fvmi_stream_synchronize(void* stream);
+int fvmi_event_create(void** event);
+int fvmi_event_destroy(void* event);
+int fvmi_event_record(void* event, void* stream);
+int fvmi_event_synchronize(void* event);
+int fvmi_event_elapsed_ms(void* start, void* end, float* ms);
+
+/* ---- memory: gpuarray.zeros / free (Common.py:510, 535-538), Memcpy2D (:543-616) */
+int fvmi_malloc(void** ptr, size_t bytes);
+/* Pitched 2-D allocation; pitch is a multiple of 256 bytes. */
+int fvmi_malloc_pitch(void** ptr, size_t* pitch, size_t width_bytes, size_t height);
+int fvmi_free(void* ptr);
+/* Page-locked host memory (cuda.pagelocked_empty, Common.py:564) */
+int fvmi_host_malloc(void** ptr, size_t bytes);
+int fvmi_host_free(void* ptr);
+int fvmi_memcpy_async(void* dst, const void* src, size_t bytes, int kind, void* stream);
+int fvmi_memcpy2d_async(void* dst, size_t dpitch, const void* src, size_t spitch, size_t width_bytes,
+                        size_t height, int kind, void* stream);
+/* Fills a width x height float32 region with `value` using a kernel (safe under
+ * stream capture, unlike a memset node). */
+int fvmi_fill_f32(float* ptr, size_t pitch_bytes, int width, int height, float value, void* stream);
+
+/* Device-to-device copy of `count` floats (16-byte aligned, count % 4 == 0)
+ * with a streaming kernel; also the known-size reference for HBM counters. */
+int fvmi_copy_f32(float* dst, const float* src, size_t count, void* stream);
+
+/* ---- substep kernels ------------------------------------------------------
+ * One dimensionally split substep from the input fields (…0) into the output
+ * fields (…1): step 0 sweeps x then y, step 1 y then x.  `dt` is the substep
+ * length (the wrappers pass dt*0.5).  Boundary fill happens on the device.
+ *
+ * `cfl`, if not NULL, points to two device floats.  The kernel atomically
+ * minimises slot (step & 1) with the per-cell admissible dt
+ * min(dx/(|u|+c), dy/(|v|+c)) of every cell it writes and sets the other slot to
+ * +inf, so after the usual step-0/step-1 pair slot 1 holds the CFL minimum of
+ * the last substep (the reference's `gpuarray.min(cfl_data)`, KP07_dimsplit.py:135-137).
+ * Before the first substep, set slot 0 to +inf and slot 1 to the initial value.
+ */
+
+/* KP07DimsplitKernel, SWE2D_KP07_dimsplit.cu:123-144 (argument order kept). */
+int fvmi_swe_kp07_dimsplit_substep(int nx, int ny, float dx, float dy, float dt, float g, float theta, int step,
+                                   int boundary_conditions, const float* h0, int h0_pitch, const float* hu0,
+                                   int hu0_pitch, const float* hv0, int hv0_pitch, float* h1, int h1_pitch,
+                                   float* hu1, int hu1_pitch, float* hv1, int hv1_pitch, float* cfl,
+                                   const fvmi_launch_t* launch, void* stream);
+
+/* HLL2Kernel, SWE2D_HLL2.cu:126-147. */
+int fvmi_swe_hll2_substep(int nx, int ny, float dx, float dy, float dt, float g, float theta, int step,
+                          int boundary_conditions, const float* h0, int h0_pitch, const float* hu0, int hu0_pitch,
+                          const float* hv0, int hv0_pitch, float* h1, int h1_pitch, float* hu1, int hu1_pitch,
+                          float* hv1, int hv1_pitch, float* cfl, const fvmi_launch_t* launch, void* stream);
+
+/* WAFKernel, SWE2D_WAF.cu:103-119.  The reference kernel has no CFL output
+ * (WAF.py:82,115 passes one it ignores); here `cfl` is filled as for KP07. */
+int fvmi_swe_waf_substep(int nx, int ny, float dx, float dy, float dt, float g, int step, int boundary_conditions,
+                         const float* h0, int h0_pitch, const float* hu0, int hu0_pitch, const float* hv0,
+                         int hv0_pitch, float* h1, int h1_pitch, float* hu1, int hu1_pitch, float* hv1,
+                         int hv1_pitch, float* cfl, const fvmi_launch_t* launch, void* stream);
+
+/* Euler KP07DimsplitKernel, EE2D_KP07_dimsplit.cu:126-154.  x0,y0,x1,y1 select
+ * an interior sub-area [x0,x1) x [y0,y1) to update (x1 == 0 -> nx, y1 == 0 -> ny,
+ * as the reference's defaults); boundary conditions are applied at the true
+ * domain edges whatever the sub-area. */
+int fvmi_euler_kp07_dimsplit_substep(int nx, int ny, float dx, float dy, float dt, float g, float gamma, float theta,
+                                     int step, int boundary_conditions, const float* rho0, int rho0_pitch,
+                                     const float* rho_u0, int rho_u0_pitch, const float* rho_v0, int rho_v0_pitch,
+                                     const float* E0, int E0_pitch, float* rho1, int rho1_pitch, float* rho_u1,
+                                     int rho_u1_pitch, float* rho_v1, int rho_v1_pitch, float* E1, int E1_pitch,
+                                     float* cfl, int x0, int y0, int x1, int y1, const fvmi_launch_t* launch,
+                                     void* stream);
+
+/* One-rank stand-in for MPISimulator.full_exchange's east/west leg
+ * (MPISimulator.py:283-295, 440-480) with a periodic-x neighbour: for each of
+ * the `nvars` fields copy padded columns [nx, nx+2) -> [0, 2) and
+ * [2, 4) -> [nx+2, nx+4) over all ny+4 rows, on the device. */
+int fvmi_halo_copy_periodic_x(int nx, int ny, int nvars, float* const* fields, const int* pitches, void* stream);
+
+/* ArakawaA2D.check (Common.py:772-779): sum of the interior (float64) and the
+ * number of non-finite interior values of one field, written to device memory
+ * (`sum_out`: 1 double, `nonfinite_out`: 1 unsigned int, both accumulated, so
+ * zero them first).  */
+int fvmi_field_stats(int nx, int ny, const float* field, int pitch, double* sum_out, unsigned int* nonfinite_out,
+                     void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FVMI_H */
