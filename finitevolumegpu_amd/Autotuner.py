@@ -159,4 +159,37 @@ class Autotuner:
         return not (np.isnan(maxval) or np.isnan(minval) or maxval > bound_max or minval < bound_min)
 
 
-__all__ = ["Autotuner", "Common"]
+def _main(argv=None):
+    """Command line sweep: prints one JSON line per configuration and the best."""
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description="Sweep the launch shape of one scheme")
+    ap.add_argument("--scheme", default="kp07", choices=["kp07", "hll2", "waf", "ee2d"])
+    ap.add_argument("--n", type=int, default=4096)
+    ap.add_argument("--tile-rows", default="16,24,32,40,48,56,60,64,68,72,80,96,128")
+    ap.add_argument("--waves-per-block", default="1,2,4")
+    ap.add_argument("--timesteps", type=int, default=20)
+    args = ap.parse_args(argv)
+    from . import KP07_dimsplit, HLL2, WAF, EE2D_KP07_dimsplit
+    cls = {"kp07": KP07_dimsplit.KP07_dimsplit, "hll2": HLL2.HLL2, "waf": WAF.WAF,
+           "ee2d": EE2D_KP07_dimsplit.EE2D_KP07_dimsplit}[args.scheme]
+    ctx = HipContext(autotuning=False)
+    data = Autotuner.gen_test_data(cls, args.n, args.n)
+    data["context"] = ctx
+    trs = [int(v) for v in args.tile_rows.split(",")]
+    wps = [int(v) for v in args.waves_per_block.split(",")]
+    best = None
+    for tr in trs:
+        for wp in wps:
+            d = dict(data)
+            d.update({"tile_rows": tr, "waves_per_block": wp})
+            mc = Autotuner.run_benchmark(cls, d, args.timesteps, 2)
+            print(json.dumps({"scheme": args.scheme, "n": args.n, "tile_rows": tr, "waves_per_block": wp,
+                              "megacells": mc}), flush=True)
+            if np.isfinite(mc) and (best is None or mc > best[0]):
+                best = (mc, tr, wp)
+    print(json.dumps({"best": {"megacells": best[0], "tile_rows": best[1], "waves_per_block": best[2]}}), flush=True)
+
+
+if __name__ == "__main__":
+    _main()

@@ -25,7 +25,7 @@ COMMON = [
 
 UNITS = [
     ("fv_kernels_exact.hip", ["-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt"]),
-    ("fv_kernels_fast.hip", ["-ffp-contract=fast"]),
+    ("fv_kernels_fast.hip", ["-ffp-contract=on"]),
     ("fvmi_runtime.hip", ["-ffp-contract=off"]),
 ]
 

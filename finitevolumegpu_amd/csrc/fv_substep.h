@@ -28,6 +28,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "fv_fast.h"
 #include "fv_math.h"
 
 namespace fvmi {
@@ -54,6 +55,8 @@ struct SubstepArgs {
     int block_cols;  // blocks along x = ceil(strips_x / waves_per_block)
     int num_blocks;  // logical blocks
     int xcd_remap;
+    // derived constants for the fast policy (set by the host launcher)
+    float hg, sqrt_g, inv_g, inv_dx, inv_dy, dtdx, dtdy;
 };
 
 struct AxisMap {
@@ -85,10 +88,10 @@ __device__ __forceinline__ AxisMap map_axis(int k, int n, int bc_lo, int bc_hi) 
 // (wave_shr:1 / wave_shl:1).  Lane 0 (resp. 63) receives 0; those lanes are
 // halo lanes whose results are never stored.
 __device__ __forceinline__ float from_left(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
 }
 __device__ __forceinline__ float from_right(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
 }
 
 __device__ __forceinline__ float wave_min(float v) {
@@ -101,42 +104,67 @@ __device__ __forceinline__ float wave_min(float v) {
 // Scheme policies
 // ---------------------------------------------------------------------------
 
+// Scheme policies.  pred_diff = F(QL) - F(QR) for the half-step predictor;
+// flux_x / flux_y the interface fluxes; cell_dt (exact) or cell_rate = 1/dt
+// (fast) the CFL quantity of an updated cell.
 template <class M>
-struct SweKP07 {
+struct SweBase {
     static constexpr int NV = 3;
-    static constexpr bool kSlopes = true;
     static constexpr bool kEuler = false;
-    __device__ static void pred_F(const float q[3], const SubstepArgs& a, float F[3]) { swe_F<M>(q, a.g, F); }
-    __device__ static void flux_x(const float l[3], const float r[3], const SubstepArgs& a, float o[3]) {
-        swe_cu_flux<M>(l, r, a.g, o);
-    }
-    __device__ static void flux_y(const float l[3], const float r[3], const SubstepArgs& a, float o[3]) {
-        swe_cu_flux<M>(l, r, a.g, o);
+    __device__ static void pred_diff(const float QL[3], const float QR[3], const SubstepArgs& a, float D[3]) {
+        if constexpr (M::exact) {
+            float FL[3], FR[3];
+            swe_F<M>(QL, a.g, FL);
+            swe_F<M>(QR, a.g, FR);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) D[k] = FL[k] - FR[k];
+        } else {
+            fast::swe_pred_diff(QL, QR, a.hg, D);
+        }
     }
     __device__ static float cell_dt(const float q[3], const SubstepArgs& a) { return swe_cell_dt<M>(q, a.dx, a.dy, a.g); }
+    __device__ static float cell_rate(const float q[3], const SubstepArgs& a) {
+        return fast::swe_cell_rate(q, a.sqrt_g, a.inv_dx, a.inv_dy);
+    }
 };
 
 template <class M>
-struct SweHLL2 {
-    static constexpr int NV = 3;
+struct SweKP07 : SweBase<M> {
     static constexpr bool kSlopes = true;
-    static constexpr bool kEuler = false;
-    __device__ static void pred_F(const float q[3], const SubstepArgs& a, float F[3]) { swe_F<M>(q, a.g, F); }
-    __device__ static void flux_x(const float l[3], const float r[3], const SubstepArgs& a, float o[3]) {
-        swe_hll_flux<M>(l, r, a.g, o);
+    __device__ static void flux(const float l[3], const float r[3], const SubstepArgs& a, float o[3]) {
+        if constexpr (M::exact)
+            swe_cu_flux<M>(l, r, a.g, o);
+        else
+            fast::swe_cu_flux(l, r, a.hg, a.sqrt_g, o);
     }
-    __device__ static void flux_y(const float l[3], const float r[3], const SubstepArgs& a, float o[3]) {
-        swe_hll_flux<M>(l, r, a.g, o);
-    }
-    __device__ static float cell_dt(const float q[3], const SubstepArgs& a) { return swe_cell_dt<M>(q, a.dx, a.dy, a.g); }
+    __device__ static void flux_x(const float l[3], const float r[3], const SubstepArgs& a, float o[3]) { flux(l, r, a, o); }
+    __device__ static void flux_y(const float l[3], const float r[3], const SubstepArgs& a, float o[3]) { flux(l, r, a, o); }
 };
 
 template <class M>
-struct SweWAF {
-    static constexpr int NV = 3;
+struct SweHLL2 : SweBase<M> {
+    static constexpr bool kSlopes = true;
+    __device__ static void flux(const float l[3], const float r[3], const SubstepArgs& a, float o[3]) {
+        if constexpr (M::exact)
+            swe_hll_flux<M>(l, r, a.g, o);
+        else
+            fast::swe_hll_flux(l, r, a.hg, a.sqrt_g, o);
+    }
+    __device__ static void flux_x(const float l[3], const float r[3], const SubstepArgs& a, float o[3]) { flux(l, r, a, o); }
+    __device__ static void flux_y(const float l[3], const float r[3], const SubstepArgs& a, float o[3]) { flux(l, r, a, o); }
+};
+
+template <class M>
+struct SweWAF : SweBase<M> {
     static constexpr bool kSlopes = false;
-    static constexpr bool kEuler = false;
-    __device__ static float cell_dt(const float q[3], const SubstepArgs& a) { return swe_cell_dt<M>(q, a.dx, a.dy, a.g); }
+    // interface flux from the four cells around it; d = dx or dy, dtd = dt/d
+    __device__ static void flux4(const float l2[3], const float l1[3], const float r1[3], const float r2[3],
+                                 const SubstepArgs& a, float d, float dtd, float o[3]) {
+        if constexpr (M::exact)
+            swe_waf_flux<M>(l2, l1, r1, r2, a.g, d, a.dt, o);
+        else
+            fast::swe_waf_flux(l2, l1, r1, r2, a.hg, a.sqrt_g, a.inv_g, dtd, o);
+    }
 };
 
 template <class M>
@@ -145,15 +173,34 @@ struct EulerKP07 {
     static constexpr bool kSlopes = true;
     static constexpr bool kEuler = true;
     // The predictor passes gamma in the pressure slot: EE2D_KP07_dimsplit.cu:57-58, 102-103.
-    __device__ static void pred_F(const float q[4], const SubstepArgs& a, float F[4]) { euler_F<M>(q, a.gamma, F); }
+    __device__ static void pred_diff(const float QL[4], const float QR[4], const SubstepArgs& a, float D[4]) {
+        if constexpr (M::exact) {
+            float FL[4], FR[4];
+            euler_F<M>(QL, a.gamma, FL);
+            euler_F<M>(QR, a.gamma, FR);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) D[k] = FL[k] - FR[k];
+        } else {
+            fast::euler_pred_diff(QL, QR, a.gamma, D);
+        }
+    }
     __device__ static void flux_x(const float l[4], const float r[4], const SubstepArgs& a, float o[4]) {
-        euler_hll_flux<M>(l, r, a.gamma, o);
+        if constexpr (M::exact)
+            euler_hll_flux<M>(l, r, a.gamma, o);
+        else
+            fast::euler_hll_flux(l, r, a.gamma, o);
     }
     __device__ static void flux_y(const float l[4], const float r[4], const SubstepArgs& a, float o[4]) {
-        euler_cu_flux<M>(l, r, a.gamma, o);
+        if constexpr (M::exact)
+            euler_cu_flux<M>(l, r, a.gamma, o);
+        else
+            fast::euler_cu_flux(l, r, a.gamma, o);
     }
     __device__ static float cell_dt(const float q[4], const SubstepArgs& a) {
         return euler_cell_dt<M>(q, a.dx, a.dy, a.gamma);
+    }
+    __device__ static float cell_rate(const float q[4], const SubstepArgs& a) {
+        return fast::euler_cell_rate(q, a.gamma, a.inv_dx, a.inv_dy);
     }
 };
 
@@ -176,28 +223,28 @@ template <class S, class M>
 __device__ __forceinline__ void reconstruct(const float l[S::NV], const float c[S::NV], const float r[S::NV],
                                             const SubstepArgs& a, float kk, float qbl[S::NV], float qbr[S::NV]) {
     constexpr int NV = S::NV;
-    float QL[NV], QR[NV], FL[NV], FR[NV];
+    float QL[NV], QR[NV], D[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
         const float s = minmod_slope<M>(l[k], c[k], r[k], a.theta);
         QL[k] = c[k] - 0.5f * s;
         QR[k] = c[k] + 0.5f * s;
     }
-    S::pred_F(QL, a, FL);
-    S::pred_F(QR, a, FR);
+    S::pred_diff(QL, QR, a, D);
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-        qbl[k] = QL[k] + kk * (FL[k] - FR[k]);
-        qbr[k] = QR[k] + kk * (FL[k] - FR[k]);
+        qbl[k] = QL[k] + kk * D[k];
+        qbr[k] = QR[k] + kk * D[k];
     }
 }
 
+// evolveF/evolveG, common.h:450, 471; dtd = dt/d (fast policy only)
 template <class M>
-__device__ __forceinline__ float evolve(float q, float f_in, float f_out, float dt, float d) {
+__device__ __forceinline__ float evolve(float q, float f_in, float f_out, float dt, float d, float dtd) {
     if constexpr (M::exact) {
-        return q + (f_in - f_out) * dt / d;  // evolveF/evolveG, common.h:450, 471
+        return q + (f_in - f_out) * dt / d;
     } else {
-        return q + (f_in - f_out) * (dt * M::rcp(d));
+        return q + (f_in - f_out) * dtd;
     }
 }
 
@@ -213,7 +260,7 @@ __device__ __forceinline__ void x_sweep_row(float q[S::NV], const SubstepArgs& a
             l[k] = from_left(q[k]);
             r[k] = from_right(q[k]);
         }
-        const float kk = M::exact ? a.dt / (2.0f * a.dx) : a.dt * M::rcp(2.0f * a.dx);
+        const float kk = M::exact ? a.dt / (2.0f * a.dx) : 0.5f * a.dtdx;
         reconstruct<S, M>(l, q, r, a, kk, qbl, qbr);
 #pragma unroll
         for (int k = 0; k < NV; ++k) qbl_r[k] = from_right(qbl[k]);
@@ -226,18 +273,25 @@ __device__ __forceinline__ void x_sweep_row(float q[S::NV], const SubstepArgs& a
             r[k] = from_right(q[k]);
             rr[k] = from_right(r[k]);
         }
-        swe_waf_flux<M>(l, q, r, rr, a.g, a.dx, a.dt, F);
+        S::flux4(l, q, r, rr, a, a.dx, a.dtdx, F);
     }
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
         const float Fl = from_left(F[k]);
-        q[k] = evolve<M>(q[k], Fl, F[k], a.dt, a.dx);
+        q[k] = evolve<M>(q[k], Fl, F[k], a.dt, a.dx, a.dtdx);
     }
 }
 
 // ---------------------------------------------------------------------------
 // The kernel
 // ---------------------------------------------------------------------------
+
+// Raw buffer descriptor of one padded field (dword3 = 32-bit data format,
+// the value used for raw buffers on gfx9).  Reads past `bytes` return 0 and
+// writes past it are dropped by the hardware.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t field_rsrc(const void* base, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
+}
 
 // Per-wave constants of a strip.
 struct Strip {
@@ -249,20 +303,46 @@ struct Strip {
     bool mirror_x;
     bool lane_out;  // lane writes its column
     int bcN, bcS;
+    __amdgpu_buffer_rsrc_t rin[4], rout[4];
 };
 
 template <int NV>
 __device__ __forceinline__ void load_row(const SubstepArgs& a, const Strip& st, int r, float v[NV]) {
-    // The row index is wave-uniform: the row base lives in scalar registers and
-    // each lane adds its 32-bit column offset.
-    const AxisMap my = map_axis(st.row0 + r, a.ny, st.bcS, st.bcN);
+    // Row offsets are wave-uniform (SGPR soffset); the lane adds its column
+    // offset (VGPR voffset): no per-lane address arithmetic.
+    const AxisMap my = map_axis(__builtin_amdgcn_readfirstlane(st.row0 + r), a.ny, st.bcS, st.bcN);
+    const int row = __builtin_amdgcn_readfirstlane(my.idx);
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-        const char* rowp = reinterpret_cast<const char*>(a.in[k]) + (size_t)my.idx * (size_t)a.in_pitch[k];
-        const float x = *reinterpret_cast<const float*>(rowp + st.lane_off);
+        const float x = __builtin_amdgcn_raw_buffer_load_b32(st.rin[k], st.lane_off, row * a.in_pitch[k], 0);
         // x mirrors flip the x momentum, y mirrors the y momentum
         const bool flip = (k == 1 && st.mirror_x) || (k == 2 && my.mirrored);
         v[k] = flip ? -x : x;
+    }
+}
+
+template <int NV>
+__device__ __forceinline__ void store_row(const SubstepArgs& a, const Strip& st, int r, const float (&q)[NV]) {
+    const int prow = __builtin_amdgcn_readfirstlane(st.row0 + r - 2);
+    if (st.lane_out) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+            __builtin_amdgcn_raw_buffer_store_b32(q[k], st.rout[k], st.col * (int)sizeof(float),
+                                                  prow * a.out_pitch[k], 0);
+    }
+}
+
+// CFL accumulator: the exact policy keeps min(dt) as the reference's writeCfl
+// does; the fast policy keeps max(1/dt) (one reciprocal per strip, taken in
+// finish_cfl).
+template <class S, class M>
+__device__ __forceinline__ void accumulate_cfl(const float (&q)[S::NV], const SubstepArgs& a, bool take, float& acc) {
+    if constexpr (M::exact) {
+        const float cd = S::cell_dt(q, a);
+        acc = take ? fminf(acc, cd) : acc;
+    } else {
+        const float rate = S::cell_rate(q, a);
+        acc = take ? fmaxf(acc, rate) : acc;
     }
 }
 
@@ -289,7 +369,7 @@ __device__ __forceinline__ void slope_row(const SubstepArgs& a, const Strip& st,
     if (!emit) return;
     float outq[NV];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) y0[k] = evolve<M>(y0[k], g_in[k], G[k], a.dt, a.dy);
+    for (int k = 0; k < NV; ++k) y0[k] = evolve<M>(y0[k], g_in[k], G[k], a.dt, a.dy, a.dtdy);
     to_y<NV>(y0, outq);  // row r-2 after the y sweep
     if constexpr (!XFirst) x_sweep_row<S, M>(outq, a);
     if constexpr (S::kEuler) {
@@ -300,15 +380,8 @@ __device__ __forceinline__ void slope_row(const SubstepArgs& a, const Strip& st,
             outq[3] -= a.g * rho_v * a.dt;
         }
     }
-    const int prow = st.row0 + r - 2;
-    if (st.lane_out) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k)
-            *reinterpret_cast<float*>(reinterpret_cast<char*>(a.out[k]) + (size_t)prow * (size_t)a.out_pitch[k] +
-                                      (size_t)st.col * sizeof(float)) = outq[k];
-    }
-    const float cd = S::cell_dt(outq, a);
-    dt_min = st.lane_out ? fminf(dt_min, cd) : dt_min;
+    store_row<NV>(a, st, r, outq);
+    accumulate_cfl<S, M>(outq, a, st.lane_out, dt_min);
 }
 
 // WAF: the y window holds 4 rows and there is no slope state.
@@ -322,24 +395,17 @@ __device__ __forceinline__ void waf_row(const SubstepArgs& a, const Strip& st, c
     to_y<3>(w1, y1);
     to_y<3>(w2, y2);
     to_y<3>(w3, y3);
-    swe_waf_flux<M>(y0, y1, y2, y3, a.g, a.dy, a.dt, G);  // interface (r-2, r-1)
+    S::flux4(y0, y1, y2, y3, a, a.dy, a.dtdy, G);  // interface (r-2, r-1)
 #pragma unroll
     for (int k = 0; k < 3; ++k) g_out[k] = G[k];
     if (!emit) return;
     float outq[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) y1[k] = evolve<M>(y1[k], g_in[k], G[k], a.dt, a.dy);
+    for (int k = 0; k < 3; ++k) y1[k] = evolve<M>(y1[k], g_in[k], G[k], a.dt, a.dy, a.dtdy);
     to_y<3>(y1, outq);
     if constexpr (!XFirst) x_sweep_row<S, M>(outq, a);
-    const int prow = st.row0 + r - 2;
-    if (st.lane_out) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-            *reinterpret_cast<float*>(reinterpret_cast<char*>(a.out[k]) + (size_t)prow * (size_t)a.out_pitch[k] +
-                                      (size_t)st.col * sizeof(float)) = outq[k];
-    }
-    const float cd = S::cell_dt(outq, a);
-    dt_min = st.lane_out ? fminf(dt_min, cd) : dt_min;
+    store_row<3>(a, st, r, outq);
+    accumulate_cfl<S, M>(outq, a, st.lane_out, dt_min);
 }
 
 template <int NV>
@@ -351,60 +417,50 @@ __device__ __forceinline__ void copyv(float (&d)[NV], const float (&s)[NV]) {
 template <class S, class M, bool XFirst>
 __device__ __forceinline__ float march_slopes(const SubstepArgs& a, const Strip& st) {
     constexpr int NV = S::NV;
-    const float kky = M::exact ? a.dt / (2.0f * a.dy) : a.dt * M::rcp(2.0f * a.dy);
-    float dt_min = __builtin_inff();
-    float A[NV], B[NV], C[NV], P0[NV], P1[NV], G0[NV], G1[NV];
+    const float kky = M::exact ? a.dt / (2.0f * a.dy) : 0.5f * a.dtdy;
+    float dt_min = M::exact ? __builtin_inff() : 0.0f;
+    // Four row slots used as a ring (the three-row window plus the row being
+    // prefetched) and two slots each for the carried face / flux; the steady
+    // loop is unrolled four times so that no slot is ever copied.
+    float A[NV], B[NV], C[NV], D[NV], P0[NV], P1[NV], G0[NV], G1[NV];
     const int last = st.r_end - 1;
     // prologue: rows 0, 1 (window fill), row 2 (first faces), row 3 (first flux)
     load_row<NV>(a, st, 0, A);
     load_row<NV>(a, st, 1, B);
+    load_row<NV>(a, st, 2, C);
+    load_row<NV>(a, st, 3, D);
     if constexpr (XFirst) {
         x_sweep_row<S, M>(A, a);
         x_sweep_row<S, M>(B, a);
+        x_sweep_row<S, M>(C, a);
     }
-    load_row<NV>(a, st, 2, C);
     {
-        // row 2: faces of row 1 only (no flux yet)
-        if constexpr (XFirst) x_sweep_row<S, M>(C, a);
         float y0[NV], y1[NV], y2[NV], qbb[NV];
         to_y<NV>(A, y0);
         to_y<NV>(B, y1);
         to_y<NV>(C, y2);
-        reconstruct<S, M>(y0, y1, y2, a, kky, qbb, P0);
+        reconstruct<S, M>(y0, y1, y2, a, kky, qbb, P0);  // faces of row 1
     }
-    // row 3: flux (1,2), no output.  Window becomes (B, C, A).
-    load_row<NV>(a, st, 3, A);
-    slope_row<S, M, XFirst>(a, st, kky, B, C, A, P0, P1, G1, G0, 3, false, dt_min);
-    // now rows (C, A) = (2, 3) hold x-swept rows 2, 3; P1 = top face of row 2; G0 = flux (1,2)
+    slope_row<S, M, XFirst>(a, st, kky, B, C, D, P0, P1, G1, G0, 3, false, dt_min);  // flux (1,2)
+    // window (C, D) = rows 2, 3; P1 = top face of row 2; G0 = flux (1,2); A <- row 4
     int r = 4;
-    float N0[NV], N1[NV];
-    load_row<NV>(a, st, min(r, last), N0);
-    for (; r + 6 <= st.r_end; r += 6) {
-        load_row<NV>(a, st, min(r + 1, last), N1);
-        copyv<NV>(B, N0);
-        slope_row<S, M, XFirst>(a, st, kky, C, A, B, P1, P0, G0, G1, r, true, dt_min);
-        load_row<NV>(a, st, min(r + 2, last), N0);
-        copyv<NV>(C, N1);
-        slope_row<S, M, XFirst>(a, st, kky, A, B, C, P0, P1, G1, G0, r + 1, true, dt_min);
-        load_row<NV>(a, st, min(r + 3, last), N1);
-        copyv<NV>(A, N0);
-        slope_row<S, M, XFirst>(a, st, kky, B, C, A, P1, P0, G0, G1, r + 2, true, dt_min);
-        load_row<NV>(a, st, min(r + 4, last), N0);
-        copyv<NV>(B, N1);
-        slope_row<S, M, XFirst>(a, st, kky, C, A, B, P0, P1, G1, G0, r + 3, true, dt_min);
-        load_row<NV>(a, st, min(r + 5, last), N1);
-        copyv<NV>(C, N0);
-        slope_row<S, M, XFirst>(a, st, kky, A, B, C, P1, P0, G0, G1, r + 4, true, dt_min);
-        load_row<NV>(a, st, min(r + 6, last), N0);
-        copyv<NV>(A, N1);
-        slope_row<S, M, XFirst>(a, st, kky, B, C, A, P0, P1, G1, G0, r + 5, true, dt_min);
+    load_row<NV>(a, st, min(r, last), A);
+    for (; r + 4 <= st.r_end; r += 4) {
+        load_row<NV>(a, st, min(r + 1, last), B);
+        slope_row<S, M, XFirst>(a, st, kky, C, D, A, P1, P0, G0, G1, r, true, dt_min);
+        load_row<NV>(a, st, min(r + 2, last), C);
+        slope_row<S, M, XFirst>(a, st, kky, D, A, B, P0, P1, G1, G0, r + 1, true, dt_min);
+        load_row<NV>(a, st, min(r + 3, last), D);
+        slope_row<S, M, XFirst>(a, st, kky, A, B, C, P1, P0, G0, G1, r + 2, true, dt_min);
+        load_row<NV>(a, st, min(r + 4, last), A);
+        slope_row<S, M, XFirst>(a, st, kky, B, C, D, P0, P1, G1, G0, r + 3, true, dt_min);
     }
-    // remainder: window (C, A), P1, G0 as at the loop head
+    // remainder (at most 3 rows): same phase as the loop head
     for (; r < st.r_end; ++r) {
-        copyv<NV>(B, N0);
-        if (r + 1 < st.r_end) load_row<NV>(a, st, r + 1, N0);
-        slope_row<S, M, XFirst>(a, st, kky, C, A, B, P1, P0, G0, G1, r, true, dt_min);
-        copyv<NV>(C, A);
+        load_row<NV>(a, st, min(r + 1, last), B);
+        slope_row<S, M, XFirst>(a, st, kky, C, D, A, P1, P0, G0, G1, r, true, dt_min);
+        copyv<NV>(C, D);
+        copyv<NV>(D, A);
         copyv<NV>(A, B);
         copyv<NV>(P1, P0);
         copyv<NV>(G0, G1);
@@ -414,20 +470,31 @@ __device__ __forceinline__ float march_slopes(const SubstepArgs& a, const Strip&
 
 template <class S, class M, bool XFirst>
 __device__ __forceinline__ float march_waf(const SubstepArgs& a, const Strip& st) {
-    float dt_min = __builtin_inff();
+    float dt_min = M::exact ? __builtin_inff() : 0.0f;
     float A[3], B[3], C[3], D[3], G0[3], G1[3];
     load_row<3>(a, st, 0, A);
     load_row<3>(a, st, 1, B);
     load_row<3>(a, st, 2, C);
+    load_row<3>(a, st, 3, D);
     if constexpr (XFirst) {
         x_sweep_row<S, M>(A, a);
         x_sweep_row<S, M>(B, a);
         x_sweep_row<S, M>(C, a);
     }
-    load_row<3>(a, st, 3, D);
     waf_row<S, M, XFirst>(a, st, A, B, C, D, G1, G0, 3, false, dt_min);  // flux (1,2)
-    // window (B, C, D) = rows 1..3, G0 = flux (1,2)
-    for (int r = 4; r < st.r_end; ++r) {
+    // window (B, C, D) = rows 1..3, G0 = flux (1,2); row r goes into the freed slot
+    int r = 4;
+    for (; r + 4 <= st.r_end; r += 4) {
+        load_row<3>(a, st, r, A);
+        waf_row<S, M, XFirst>(a, st, B, C, D, A, G0, G1, r, true, dt_min);
+        load_row<3>(a, st, r + 1, B);
+        waf_row<S, M, XFirst>(a, st, C, D, A, B, G1, G0, r + 1, true, dt_min);
+        load_row<3>(a, st, r + 2, C);
+        waf_row<S, M, XFirst>(a, st, D, A, B, C, G0, G1, r + 2, true, dt_min);
+        load_row<3>(a, st, r + 3, D);
+        waf_row<S, M, XFirst>(a, st, A, B, C, D, G1, G0, r + 3, true, dt_min);
+    }
+    for (; r < st.r_end; ++r) {
         load_row<3>(a, st, r, A);
         waf_row<S, M, XFirst>(a, st, B, C, D, A, G0, G1, r, true, dt_min);
         copyv<3>(B, C);
@@ -441,7 +508,8 @@ __device__ __forceinline__ float march_waf(const SubstepArgs& a, const Strip& st
 template <class S, class M>
 __global__ void __launch_bounds__(256) substep_kernel(SubstepArgs a) {
     const int lane = threadIdx.x & (kLanes - 1);
-    const int wave = threadIdx.x >> 6;
+    // wave index made scalar so that every row address is computed in SGPRs
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (a.cfl != nullptr && blockIdx.x == 0 && threadIdx.x == 0) a.cfl[1 - (a.step & 1)] = __builtin_inff();
     // Hardware dispatch places workgroup i on XCD i % 8; remap so that each XCD
     // owns a contiguous range of logical blocks (neighbouring strips share L2).
@@ -458,14 +526,19 @@ __global__ void __launch_bounds__(256) substep_kernel(SubstepArgs a) {
     Strip st;
     st.lane = lane;
     st.col = a.x0 + kStripCols * strip_x + lane;
-    st.row0 = a.y0 + a.tile_rows * strip_y;
-    st.r_end = min(a.tile_rows + 4, a.y1 + 4 - st.row0);  // never past padded row y1+3 <= ny+3
+    st.row0 = __builtin_amdgcn_readfirstlane(a.y0 + a.tile_rows * strip_y);
+    st.r_end = __builtin_amdgcn_readfirstlane(min(a.tile_rows + 4, a.y1 + 4 - st.row0));  // never past padded row y1+3 <= ny+3
     st.bcN = (a.bc >> 24) & 0xF;
     st.bcS = (a.bc >> 16) & 0xF;
     const AxisMap mx = map_axis(st.col, a.nx, a.bc & 0xF, (a.bc >> 8) & 0xF);
     st.lane_off = mx.idx * (int)sizeof(float);
     st.mirror_x = mx.mirrored;
     st.lane_out = lane >= 2 && lane < 2 + kStripCols && st.col < a.x1 + 2;
+#pragma unroll
+    for (int k = 0; k < S::NV; ++k) {
+        st.rin[k] = field_rsrc(a.in[k], (unsigned)(a.ny + 4) * (unsigned)a.in_pitch[k]);
+        st.rout[k] = field_rsrc(a.out[k], (unsigned)(a.ny + 4) * (unsigned)a.out_pitch[k]);
+    }
 
     float dt_min;
     if constexpr (S::kSlopes) {
@@ -473,6 +546,7 @@ __global__ void __launch_bounds__(256) substep_kernel(SubstepArgs a) {
     } else {
         dt_min = (a.step == 0) ? march_waf<S, M, true>(a, st) : march_waf<S, M, false>(a, st);
     }
+    if constexpr (!M::exact) dt_min = (dt_min > 0.0f) ? fast::rcp(dt_min) : __builtin_inff();
     if (a.cfl != nullptr) {
         const float m = wave_min(dt_min);
         if (lane == 0 && m < __builtin_inff())

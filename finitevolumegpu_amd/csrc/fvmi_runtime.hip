@@ -71,7 +71,7 @@ int fill_launch(const fvmi_launch_t* l, SubstepArgs& a, int* math) {
 
 int run_substep(int scheme, SubstepArgs& a, int nvars, const fvmi_launch_t* launch, void* stream) {
     if (a.nx < 2 || a.ny < 2) return fail(FVMI_ERR_INVALID, "grid %dx%d too small (need nx, ny >= 2)", a.nx, a.ny);
-    if ((size_t)(a.ny + 4) * (size_t)(a.nx + 4) > (size_t)1 << 34)
+    if ((size_t)(a.ny + 4) * (size_t)(a.nx + 4) > (size_t)1 << 30)
         return fail(FVMI_ERR_INVALID, "grid %dx%d too large", a.nx, a.ny);
     if (a.step != 0 && a.step != 1) return fail(FVMI_ERR_INVALID, "step must be 0 or 1, got %d", a.step);
     if (int rc = check_bc(a.bc)) return rc;
@@ -83,6 +83,10 @@ int run_substep(int scheme, SubstepArgs& a, int nvars, const fvmi_launch_t* laun
         if (a.in[k] == a.out[k]) return fail(FVMI_ERR_INVALID, "field %d: input and output alias", k);
         if (a.in_pitch[k] < min_pitch || a.out_pitch[k] < min_pitch || (a.in_pitch[k] & 3) || (a.out_pitch[k] & 3))
             return fail(FVMI_ERR_INVALID, "field %d: pitch must be >= %d bytes and a multiple of 4", k, min_pitch);
+        // fields are addressed through 32-bit buffer descriptors
+        if ((size_t)(a.ny + 4) * (size_t)a.in_pitch[k] >= ((size_t)1 << 32) ||
+            (size_t)(a.ny + 4) * (size_t)a.out_pitch[k] >= ((size_t)1 << 32))
+            return fail(FVMI_ERR_INVALID, "field %d: (ny+4)*pitch must be below 4 GiB", k);
     }
     if (a.x1 == 0) a.x1 = a.nx;
     if (a.y1 == 0) a.y1 = a.ny;
@@ -91,6 +95,14 @@ int run_substep(int scheme, SubstepArgs& a, int nvars, const fvmi_launch_t* laun
                     a.nx, a.ny);
     int math = 0;
     if (int rc = fill_launch(launch, a, &math)) return rc;
+    // constants of the fast policy, rounded once on the host
+    a.hg = 0.5f * a.g;
+    a.sqrt_g = std::sqrt(a.g > 0.0f ? a.g : 0.0f);
+    a.inv_g = a.g != 0.0f ? 1.0f / a.g : 0.0f;
+    a.inv_dx = 1.0f / a.dx;
+    a.inv_dy = 1.0f / a.dy;
+    a.dtdx = a.dt / a.dx;
+    a.dtdy = a.dt / a.dy;
     hipError_t e = (math == FVMI_MATH_EXACT) ? launch_substep_exact(scheme, a, (hipStream_t)stream)
                                              : launch_substep_fast(scheme, a, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "substep launch");

@@ -76,5 +76,5 @@ def test_launch_shapes_agree(ctx, tile_rows, waves):
         for _ in range(3):
             ref.step(np.float32(0.05))
             sim.step(np.float32(0.05))
-        for a, b in zip(sim.download(), ref.download()):
-            assert np.array_equal(a, b)
+        for k, (a, b) in enumerate(zip(sim.download(), ref.download())):
+            assert np.array_equal(a, b), (math, k, float(np.abs(a - b).max()))
