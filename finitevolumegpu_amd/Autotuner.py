@@ -166,7 +166,7 @@ def _main(argv=None):
     ap = argparse.ArgumentParser(description="Sweep the launch shape of one scheme")
     ap.add_argument("--scheme", default="kp07", choices=["kp07", "hll2", "waf", "ee2d"])
     ap.add_argument("--n", type=int, default=4096)
-    ap.add_argument("--tile-rows", default="16,24,32,40,48,56,60,64,68,72,80,96,128")
+    ap.add_argument("--tile-rows", default="24,32,40,48,56,60,64,68,70,72,80,96,128")
     ap.add_argument("--waves-per-block", default="1,2,4")
     ap.add_argument("--timesteps", type=int, default=20)
     args = ap.parse_args(argv)
@@ -188,8 +188,12 @@ def _main(argv=None):
                               "megacells": mc}), flush=True)
             if np.isfinite(mc) and (best is None or mc > best[0]):
                 best = (mc, tr, wp)
+    if best is None:
+        print(json.dumps({"best": None}), flush=True)
+        return 1
     print(json.dumps({"best": {"megacells": best[0], "tile_rows": best[1], "waves_per_block": best[2]}}), flush=True)
+    return 0
 
 
 if __name__ == "__main__":
-    _main()
+    raise SystemExit(_main())

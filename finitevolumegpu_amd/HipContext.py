@@ -78,6 +78,9 @@ class HipContext:
         buf = ctypes.create_string_buffer(256)
         _lib.call("fvmi_device_name", self.device, buf, 256)
         self.device_name = buf.value.decode(errors="replace")
+        cus = ctypes.c_int()
+        _lib.call("fvmi_device_cu_count", self.device, ctypes.byref(cus))
+        self.cu_count = cus.value
         free, total = self.mem_info()
         self.logger.info("Using device %d '%s', %d / %d MB free", self.device, self.device_name,
                          free // (1 << 20), total // (1 << 20))

@@ -42,6 +42,7 @@ SIGNATURES = {
     "fvmi_ctx_synchronize": (c_int, [c_void_p]),
     "fvmi_ctx_device": (c_int, [c_void_p, P(c_int)]),
     "fvmi_device_name": (c_int, [c_int, c_char_p, c_int]),
+    "fvmi_device_cu_count": (c_int, [c_int, P(c_int)]),
     "fvmi_mem_get_info": (c_int, [P(c_size_t), P(c_size_t)]),
     "fvmi_stream_create": (c_int, [P(c_void_p)]),
     "fvmi_stream_destroy": (c_int, [c_void_p]),

@@ -314,7 +314,9 @@ __device__ __forceinline__ void load_row(const SubstepArgs& a, const Strip& st, 
     const int row = __builtin_amdgcn_readfirstlane(my.idx);
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-        const float x = __builtin_amdgcn_raw_buffer_load_b32(st.rin[k], st.lane_off, row * a.in_pitch[k], 0);
+        // the b32 buffer builtins move raw 32-bit words
+        const float x =
+            __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(st.rin[k], st.lane_off, row * a.in_pitch[k], 0));
         // x mirrors flip the x momentum, y mirrors the y momentum
         const bool flip = (k == 1 && st.mirror_x) || (k == 2 && my.mirrored);
         v[k] = flip ? -x : x;
@@ -327,7 +329,7 @@ __device__ __forceinline__ void store_row(const SubstepArgs& a, const Strip& st,
     if (st.lane_out) {
 #pragma unroll
         for (int k = 0; k < NV; ++k)
-            __builtin_amdgcn_raw_buffer_store_b32(q[k], st.rout[k], st.col * (int)sizeof(float),
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(q[k]), st.rout[k], st.col * (int)sizeof(float),
                                                   prow * a.out_pitch[k], 0);
     }
 }

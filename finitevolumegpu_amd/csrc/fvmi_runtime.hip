@@ -220,6 +220,14 @@ int fvmi_device_name(int device, char* buf, int len) {
     return FVMI_OK;
 }
 
+int fvmi_device_cu_count(int device, int* cus) {
+    if (!cus) return fail(FVMI_ERR_INVALID, "cus is NULL");
+    hipDeviceProp_t p;
+    FVMI_HIP(hipGetDeviceProperties(&p, device));
+    *cus = p.multiProcessorCount;
+    return FVMI_OK;
+}
+
 int fvmi_mem_get_info(size_t* free_bytes, size_t* total_bytes) {
     if (!free_bytes || !total_bytes) return fail(FVMI_ERR_INVALID, "NULL argument");
     FVMI_HIP(hipMemGetInfo(free_bytes, total_bytes));

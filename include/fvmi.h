@@ -68,6 +68,8 @@ int fvmi_ctx_make_current(void* ctx);
 int fvmi_ctx_synchronize(void* ctx);
 int fvmi_ctx_device(void* ctx, int* device);
 int fvmi_device_name(int device, char* buf, int len);
+/* Number of compute units (256 on MI355X); used to size launches. */
+int fvmi_device_cu_count(int device, int* cus);
 /* cuda.mem_get_info(), CudaContext.py:81 */
 int fvmi_mem_get_info(size_t* free_bytes, size_t* total_bytes);
 
