@@ -173,41 +173,12 @@ class BaseSimulator:
 
 _MATH = {"fast": _lib.FVMI_MATH_FAST, "exact": _lib.FVMI_MATH_EXACT}
 
-STRIP_COLS = 60  # output columns of one 64-lane strip (fv_substep.h kStripCols)
-
-
-def choose_tile_rows(nx, ny, cu_count, simds_per_cu=4, resident_waves=5, busy_waves=4, lo=24, hi=128):
-    """Rows per strip that balance the launch.  Minimises the product of
-    * the halo-row overhead (the first sweep covers tile_rows+4 rows on every
-      other substep),
-    * the tail loss of the last, partly filled round of waves over the SIMDs,
-    * a latency penalty when fewer than ``busy_waves`` waves share a SIMD."""
-    strips_x = -(-int(nx) // STRIP_COLS)
-    slots = max(1, int(cu_count) * simds_per_cu)
-    best = None
-    for h in range(lo, hi + 1):
-        waves = strips_x * -(-int(ny) // h)
-        per_simd = waves / slots
-        if per_simd <= resident_waves:
-            tail = np.ceil(per_simd) / per_simd
-        else:
-            rounds = per_simd / resident_waves
-            tail = np.ceil(rounds) / rounds
-        resident = min(per_simd, resident_waves)
-        latency = busy_waves / resident if resident < busy_waves else 1.0
-        cost = 0.5 * ((h + 4.0) / h + 1.0) * tail * latency
-        if best is None or cost < best[0] - 1e-9:
-            best = (cost, h)
-    return best[1]
-
-
 class DimsplitSimulator(BaseSimulator):
     """Shared body of the four dimensionally split scheme wrappers."""
 
     scheme = None  # "kp07_dimsplit" | "hll2" | "waf" | "ee2d_kp07_dimsplit"
     nvars = 3
     # default launch shape when neither the caller nor the autotuner gives one
-    default_tile_rows = 64
     default_waves_per_block = 4
 
     def _init_dimsplit(self, fields, cfl_host, tile_rows, waves_per_block, math, xcd_remap):
@@ -219,9 +190,9 @@ class DimsplitSimulator(BaseSimulator):
         if math not in _MATH:
             raise ValueError("math must be 'fast' or 'exact'")
         self.math = math
-        if tile_rows is None:
-            tile_rows = choose_tile_rows(self.nx, self.ny, getattr(self.context, "cu_count", 256))
-        self.launch = _lib.LaunchConfig(int(tile_rows),
+        # tile_rows = rows per wavefront segment; 0 lets the library balance the
+        # segments over the device's SIMDs
+        self.launch = _lib.LaunchConfig(int(tile_rows or 0),
                                         int(waves_per_block or self.default_waves_per_block),
                                         _MATH[math], 1 if xcd_remap else 2)
         self.gc_x = 2

@@ -4,7 +4,7 @@
 #include "fv_launch.h"
 
 namespace fvmi {
-hipError_t launch_substep_exact(int scheme, const SubstepArgs& a, hipStream_t stream) {
-    return launch_substep<MathExact>(scheme, a, stream);
+hipError_t launch_substep_exact(int scheme, const SubstepArgs& a, int grid, hipStream_t stream) {
+    return launch_substep<MathExact>(scheme, a, grid, stream);
 }
 }  // namespace fvmi

@@ -3,7 +3,7 @@
 #include "fv_launch.h"
 
 namespace fvmi {
-hipError_t launch_substep_fast(int scheme, const SubstepArgs& a, hipStream_t stream) {
-    return launch_substep<MathFast>(scheme, a, stream);
+hipError_t launch_substep_fast(int scheme, const SubstepArgs& a, int grid, hipStream_t stream) {
+    return launch_substep<MathFast>(scheme, a, grid, stream);
 }
 }  // namespace fvmi

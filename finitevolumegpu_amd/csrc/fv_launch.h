@@ -9,21 +9,30 @@ namespace fvmi {
 
 enum SchemeId : int { SCHEME_KP07 = 0, SCHEME_HLL2 = 1, SCHEME_WAF = 2, SCHEME_EULER = 3 };
 
-// Fills the decomposition fields of `a` and returns the grid size to launch.
-inline int plan_substep(SubstepArgs& a) {
-    const int wx = a.x1 - a.x0, wy = a.y1 - a.y0;
+// Fills the decomposition fields of `a` (sub-area, launch shape, device size)
+// and returns the grid size to launch.  seg_rows == 0 picks the balanced
+// segment length: about `waves_per_simd` waves on every SIMD of the device,
+// but no segment shorter than `min_seg` rows (the 4-row prologue of a segment
+// is then at most 1/6 of its work).
+inline int plan_substep(SubstepArgs& a, int cu_count, int waves_per_simd = 4, int min_seg = 24) {
+    const int wx = a.x1 - a.x0;
+    a.rows_per_strip = a.y1 - a.y0;
     a.strips_x = (wx + kStripCols - 1) / kStripCols;
-    a.strips_y = (wy + a.tile_rows - 1) / a.tile_rows;
-    a.block_cols = (a.strips_x + a.waves_per_block - 1) / a.waves_per_block;
-    a.num_blocks = a.block_cols * a.strips_y;
+    a.total_rows = a.strips_x * a.rows_per_strip;
+    if (a.seg_rows <= 0) {
+        const int target = cu_count * 4 * waves_per_simd;
+        int seg = (a.total_rows + target - 1) / target;
+        a.seg_rows = seg < min_seg ? min_seg : seg;
+    }
+    a.num_waves = (a.total_rows + a.seg_rows - 1) / a.seg_rows;
+    a.num_blocks = (a.num_waves + a.waves_per_block - 1) / a.waves_per_block;
     int grid = a.num_blocks;
     if (a.xcd_remap) grid = ((grid + 7) / 8) * 8;
     return grid;
 }
 
 template <class M>
-hipError_t launch_substep(int scheme, SubstepArgs a, hipStream_t stream) {
-    const int grid = plan_substep(a);
+hipError_t launch_substep(int scheme, SubstepArgs a, int grid, hipStream_t stream) {
     const dim3 block(kLanes * a.waves_per_block);
     switch (scheme) {
         case SCHEME_KP07:
@@ -45,7 +54,7 @@ hipError_t launch_substep(int scheme, SubstepArgs a, hipStream_t stream) {
 }
 
 // Defined in fv_kernels_exact.hip / fv_kernels_fast.hip.
-hipError_t launch_substep_exact(int scheme, const SubstepArgs& a, hipStream_t stream);
-hipError_t launch_substep_fast(int scheme, const SubstepArgs& a, hipStream_t stream);
+hipError_t launch_substep_exact(int scheme, const SubstepArgs& a, int grid, hipStream_t stream);
+hipError_t launch_substep_fast(int scheme, const SubstepArgs& a, int grid, hipStream_t stream);
 
 }  // namespace fvmi

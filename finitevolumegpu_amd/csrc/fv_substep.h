@@ -49,11 +49,17 @@ struct SubstepArgs {
     float* out[4];
     int out_pitch[4];
     float* cfl;  // two-slot CFL minimum (slot = step&1 written, the other reset); may be null
-    int tile_rows;
-    int strips_x, strips_y;
+    // Work decomposition: the sub-area is cut into 60-column strips; the
+    // strips' rows, laid end to end (strip after strip), are split into equal
+    // segments of `seg_rows` rows, one per wavefront, so every wave has the
+    // same amount of work (a segment may continue into the next strip).
+    int seg_rows;
+    int strips_x;
+    int rows_per_strip;  // y1 - y0
+    int total_rows;      // strips_x * rows_per_strip
+    int num_waves;
     int waves_per_block;
-    int block_cols;  // blocks along x = ceil(strips_x / waves_per_block)
-    int num_blocks;  // logical blocks
+    int num_blocks;  // logical blocks = ceil(num_waves / waves_per_block)
     int xcd_remap;
     // derived constants for the fast policy (set by the host launcher)
     float hg, sqrt_g, inv_g, inv_dx, inv_dy, dtdx, dtdy;
@@ -508,6 +514,26 @@ __device__ __forceinline__ float march_waf(const SubstepArgs& a, const Strip& st
 }
 
 template <class S, class M>
+__device__ __forceinline__ void setup_strip(const SubstepArgs& a, Strip& st, int lane, int strip_x, int row_in,
+                                            int count) {
+    st.lane = lane;
+    st.col = a.x0 + kStripCols * strip_x + lane;
+    st.row0 = __builtin_amdgcn_readfirstlane(a.y0 + row_in);  // padded row of r = 0
+    st.r_end = __builtin_amdgcn_readfirstlane(count + 4);     // output rows r = 4 .. count+3
+    st.bcN = (a.bc >> 24) & 0xF;
+    st.bcS = (a.bc >> 16) & 0xF;
+    const AxisMap mx = map_axis(st.col, a.nx, a.bc & 0xF, (a.bc >> 8) & 0xF);
+    st.lane_off = mx.idx * (int)sizeof(float);
+    st.mirror_x = mx.mirrored;
+    st.lane_out = lane >= 2 && lane < 2 + kStripCols && st.col < a.x1 + 2;
+#pragma unroll
+    for (int k = 0; k < S::NV; ++k) {
+        st.rin[k] = field_rsrc(a.in[k], (unsigned)(a.ny + 4) * (unsigned)a.in_pitch[k]);
+        st.rout[k] = field_rsrc(a.out[k], (unsigned)(a.ny + 4) * (unsigned)a.out_pitch[k]);
+    }
+}
+
+template <class S, class M>
 __global__ void __launch_bounds__(256) substep_kernel(SubstepArgs a) {
     const int lane = threadIdx.x & (kLanes - 1);
     // wave index made scalar so that every row address is computed in SGPRs
@@ -520,35 +546,29 @@ __global__ void __launch_bounds__(256) substep_kernel(SubstepArgs a) {
         const int per = gridDim.x >> 3;  // gridDim.x is a multiple of 8 when remapping
         lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     }
-    if (lb >= a.num_blocks) return;
-    const int strip_x = (lb % a.block_cols) * a.waves_per_block + wave;
-    const int strip_y = lb / a.block_cols;
-    if (strip_x >= a.strips_x) return;
+    const int gw = __builtin_amdgcn_readfirstlane(lb * a.waves_per_block + wave);
+    if (lb >= a.num_blocks || gw >= a.num_waves) return;
 
-    Strip st;
-    st.lane = lane;
-    st.col = a.x0 + kStripCols * strip_x + lane;
-    st.row0 = __builtin_amdgcn_readfirstlane(a.y0 + a.tile_rows * strip_y);
-    st.r_end = __builtin_amdgcn_readfirstlane(min(a.tile_rows + 4, a.y1 + 4 - st.row0));  // never past padded row y1+3 <= ny+3
-    st.bcN = (a.bc >> 24) & 0xF;
-    st.bcS = (a.bc >> 16) & 0xF;
-    const AxisMap mx = map_axis(st.col, a.nx, a.bc & 0xF, (a.bc >> 8) & 0xF);
-    st.lane_off = mx.idx * (int)sizeof(float);
-    st.mirror_x = mx.mirrored;
-    st.lane_out = lane >= 2 && lane < 2 + kStripCols && st.col < a.x1 + 2;
-#pragma unroll
-    for (int k = 0; k < S::NV; ++k) {
-        st.rin[k] = field_rsrc(a.in[k], (unsigned)(a.ny + 4) * (unsigned)a.in_pitch[k]);
-        st.rout[k] = field_rsrc(a.out[k], (unsigned)(a.ny + 4) * (unsigned)a.out_pitch[k]);
+    float acc = M::exact ? __builtin_inff() : 0.0f;
+    int pos = gw * a.seg_rows;
+    const int end = min(pos + a.seg_rows, a.total_rows);
+    while (pos < end) {  // at most two segments in practice
+        const int strip_x = pos / a.rows_per_strip;
+        const int row_in = pos - strip_x * a.rows_per_strip;
+        const int count = min(end - pos, a.rows_per_strip - row_in);
+        Strip st;
+        setup_strip<S, M>(a, st, lane, strip_x, row_in, count);
+        float part;
+        if constexpr (S::kSlopes) {
+            part = (a.step == 0) ? march_slopes<S, M, true>(a, st) : march_slopes<S, M, false>(a, st);
+        } else {
+            part = (a.step == 0) ? march_waf<S, M, true>(a, st) : march_waf<S, M, false>(a, st);
+        }
+        acc = M::exact ? fminf(acc, part) : fmaxf(acc, part);
+        pos += count;
     }
-
-    float dt_min;
-    if constexpr (S::kSlopes) {
-        dt_min = (a.step == 0) ? march_slopes<S, M, true>(a, st) : march_slopes<S, M, false>(a, st);
-    } else {
-        dt_min = (a.step == 0) ? march_waf<S, M, true>(a, st) : march_waf<S, M, false>(a, st);
-    }
-    if constexpr (!M::exact) dt_min = (dt_min > 0.0f) ? fast::rcp(dt_min) : __builtin_inff();
+    float dt_min = acc;
+    if constexpr (!M::exact) dt_min = (acc > 0.0f) ? fast::rcp(acc) : __builtin_inff();
     if (a.cfl != nullptr) {
         const float m = wave_min(dt_min);
         if (lane == 0 && m < __builtin_inff())

@@ -38,6 +38,20 @@ struct Ctx {
     int device;
 };
 
+// Compute-unit count of the current device, cached per device ordinal.
+int current_cu_count() {
+    static int cache[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (cache[dev] == 0) {
+        hipDeviceProp_t p;
+        cache[dev] = (hipGetDeviceProperties(&p, dev) == hipSuccess && p.multiProcessorCount > 0)
+                         ? p.multiProcessorCount
+                         : 256;
+    }
+    return cache[dev];
+}
+
 hipMemcpyKind kind_of(int kind) {
     switch (kind) {
         case FVMI_MEMCPY_HTOD: return hipMemcpyHostToDevice;
@@ -59,11 +73,11 @@ int check_bc(int bc) {
 }
 
 int fill_launch(const fvmi_launch_t* l, SubstepArgs& a, int* math) {
-    a.tile_rows = (l && l->tile_rows > 0) ? l->tile_rows : 64;
+    a.seg_rows = (l && l->tile_rows > 0) ? l->tile_rows : 0;  // 0 = balanced
     a.waves_per_block = (l && l->waves_per_block > 0) ? l->waves_per_block : 4;
     a.xcd_remap = (l && l->xcd_remap == 2) ? 0 : 1;
     *math = l ? l->math : FVMI_MATH_FAST;
-    if (a.tile_rows > 4096) return fail(FVMI_ERR_INVALID, "tile_rows %d out of range [1, 4096]", a.tile_rows);
+    if (a.seg_rows > (1 << 20)) return fail(FVMI_ERR_INVALID, "tile_rows %d out of range", a.seg_rows);
     if (a.waves_per_block > 4) return fail(FVMI_ERR_INVALID, "waves_per_block %d out of range [1, 4]", a.waves_per_block);
     if (*math != FVMI_MATH_FAST && *math != FVMI_MATH_EXACT) return fail(FVMI_ERR_INVALID, "unknown math mode %d", *math);
     return FVMI_OK;
@@ -103,8 +117,9 @@ int run_substep(int scheme, SubstepArgs& a, int nvars, const fvmi_launch_t* laun
     a.inv_dy = 1.0f / a.dy;
     a.dtdx = a.dt / a.dx;
     a.dtdy = a.dt / a.dy;
-    hipError_t e = (math == FVMI_MATH_EXACT) ? launch_substep_exact(scheme, a, (hipStream_t)stream)
-                                             : launch_substep_fast(scheme, a, (hipStream_t)stream);
+    const int grid = plan_substep(a, current_cu_count());
+    hipError_t e = (math == FVMI_MATH_EXACT) ? launch_substep_exact(scheme, a, grid, (hipStream_t)stream)
+                                             : launch_substep_fast(scheme, a, grid, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "substep launch");
     return FVMI_OK;
 }

@@ -48,7 +48,7 @@ extern "C" {
  * the default.  Replaces the compile-time BLOCK_WIDTH/BLOCK_HEIGHT defines of
  * the reference kernels (KP07_dimsplit.py:80-89). */
 typedef struct fvmi_launch {
-    int tile_rows;       /* output rows per 64-lane column strip (default 64) */
+    int tile_rows;       /* rows per wavefront segment; 0 (default) = balanced over the device */
     int waves_per_block; /* strips per workgroup, 1..4 (default 4) */
     int math;            /* FVMI_MATH_FAST (default) or FVMI_MATH_EXACT */
     int xcd_remap;       /* 1 (default): consecutive strips on one XCD; 2: off */
