@@ -139,19 +139,47 @@ struct HaloArgs {
     float* f[4];
     int pitch[4];
     int nx, ny, nvars;
+    int wrap_x, wrap_y;
 };
 
-// thread = (row, var, one of the 4 ghost columns)
-__global__ void halo_periodic_x_kernel(HaloArgs a) {
-    const int row = blockIdx.x * blockDim.y + threadIdx.y;
-    const int c = threadIdx.x & 3;
-    const int var = threadIdx.x >> 2;
-    if (row >= a.ny + 4 || var >= a.nvars) return;
-    float* r = reinterpret_cast<float*>(reinterpret_cast<char*>(a.f[var]) + (size_t)row * a.pitch[var]);
-    // destination columns 0,1 <- nx, nx+1 ; nx+2, nx+3 <- 2, 3
-    const int dst = (c < 2) ? c : a.nx + c;
-    const int src = (c < 2) ? a.nx + c : c;
-    r[dst] = r[src];
+// One thread per ghost-frame cell and field.  The source of every ghost cell is
+// read from memory that this launch does not write, so the exchange is a single
+// race-free pass: north/south rows over the interior columns, then east/west
+// columns over the full height with the corners taken from the wrapped rows,
+// the order of MPISimulator.full_exchange (MPISimulator.py:371-480).
+__global__ void halo_periodic_kernel(HaloArgs a) {
+    const int W = a.nx + 4;
+    const int frame = 2 * W * 2 + 2 * 2 * a.ny;  // ghost rows (full width) + ghost columns (interior rows)
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int var = blockIdx.y;
+    if (t >= frame || var >= a.nvars) return;
+    int r, c;
+    if (t < 4 * W) {  // rows 0,1,ny+2,ny+3
+        const int k = t / W;
+        r = (k < 2) ? k : a.ny + k;
+        c = t - k * W;
+    } else {  // columns 0,1,nx+2,nx+3 of the interior rows
+        const int u = t - 4 * W;
+        const int k = u / a.ny;
+        r = 2 + (u - k * a.ny);
+        c = (k < 2) ? k : a.nx + k;
+    }
+    const bool ghost_c = c < 2 || c >= a.nx + 2;
+    const bool ghost_r = r < 2 || r >= a.ny + 2;
+    int sr = r, sc = c;
+    if (ghost_c) {
+        // east/west leg: full height, rows as they stand after the north/south leg
+        if (!a.wrap_x) return;
+        sc = (c < 2) ? c + a.nx : c - a.nx;
+        if (ghost_r && a.wrap_y) sr = (r < 2) ? r + a.ny : r - a.ny;
+    } else {
+        // north/south leg: interior columns only
+        if (!a.wrap_y) return;
+        sr = (r < 2) ? r + a.ny : r - a.ny;
+    }
+    char* base = reinterpret_cast<char*>(a.f[var]);
+    const float v = reinterpret_cast<const float*>(base + (size_t)sr * a.pitch[var])[sc];
+    reinterpret_cast<float*>(base + (size_t)r * a.pitch[var])[c] = v;
 }
 
 __global__ void field_stats_kernel(int nx, int ny, const float* f, int pitch, double* sum_out, unsigned int* bad_out) {
@@ -427,23 +455,27 @@ int fvmi_euler_kp07_dimsplit_substep(int nx, int ny, float dx, float dy, float d
     return run_substep(SCHEME_EULER, a, 4, launch, stream);
 }
 
-int fvmi_halo_copy_periodic_x(int nx, int ny, int nvars, float* const* fields, const int* pitches, void* stream) {
-    if (nx < 2 || ny < 0 || nvars < 1 || nvars > 4 || !fields || !pitches)
-        return fail(FVMI_ERR_INVALID, "bad halo copy arguments");
+int fvmi_halo_exchange_periodic(int nx, int ny, int nvars, float* const* fields, const int* pitches, int wrap_x,
+                                int wrap_y, void* stream) {
+    if (nx < 2 || ny < 2 || nvars < 1 || nvars > 4 || !fields || !pitches)
+        return fail(FVMI_ERR_INVALID, "bad halo exchange arguments");
     HaloArgs a{};
-    a.nx = nx, a.ny = ny, a.nvars = nvars;
+    a.nx = nx, a.ny = ny, a.nvars = nvars, a.wrap_x = wrap_x ? 1 : 0, a.wrap_y = wrap_y ? 1 : 0;
     for (int k = 0; k < nvars; ++k) {
         if (!fields[k] || pitches[k] < (nx + 4) * (int)sizeof(float))
-            return fail(FVMI_ERR_INVALID, "bad field %d for halo copy", k);
+            return fail(FVMI_ERR_INVALID, "bad field %d for halo exchange", k);
         a.f[k] = fields[k];
         a.pitch[k] = pitches[k];
     }
-    const int rows_per_block = 16;
-    const dim3 block(16, rows_per_block);  // x: 4 vars x 4 ghost columns
-    const dim3 grid((ny + 4 + rows_per_block - 1) / rows_per_block);
-    hipLaunchKernelGGL(halo_periodic_x_kernel, grid, block, 0, (hipStream_t)stream, a);
+    if (!a.wrap_x && !a.wrap_y) return FVMI_OK;
+    const int frame = 4 * (nx + 4) + 4 * ny;
+    hipLaunchKernelGGL(halo_periodic_kernel, dim3((frame + 255) / 256, nvars), dim3(256), 0, (hipStream_t)stream, a);
     FVMI_HIP(hipGetLastError());
     return FVMI_OK;
+}
+
+int fvmi_halo_copy_periodic_x(int nx, int ny, int nvars, float* const* fields, const int* pitches, void* stream) {
+    return fvmi_halo_exchange_periodic(nx, ny, nvars, fields, pitches, 1, 0, stream);
 }
 
 int fvmi_field_stats(int nx, int ny, const float* field, int pitch, double* sum_out, unsigned int* nonfinite_out,

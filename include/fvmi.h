@@ -153,6 +153,14 @@ int fvmi_euler_kp07_dimsplit_substep(int nx, int ny, float dx, float dy, float d
  * [2, 4) -> [nx+2, nx+4) over all ny+4 rows, on the device. */
 int fvmi_halo_copy_periodic_x(int nx, int ny, int nvars, float* const* fields, const int* pitches, void* stream);
 
+/* The same exchange for a rank that is its own neighbour in x and/or y
+ * (MPIGrid with one rank and periodic edges, MPISimulator.py:247-309): with
+ * wrap_y, ghost rows [0,2) <- [ny,ny+2) and [ny+2,ny+4) <- [2,4) over the
+ * interior columns; with wrap_x, the ghost columns over all ny+4 rows, corners
+ * taken after the row exchange (the reference's N/S-then-E/W order). */
+int fvmi_halo_exchange_periodic(int nx, int ny, int nvars, float* const* fields, const int* pitches, int wrap_x,
+                                int wrap_y, void* stream);
+
 /* ArakawaA2D.check (Common.py:772-779): sum of the interior (float64) and the
  * number of non-finite interior values of one field, written to device memory
  * (`sum_out`: 1 double, `nonfinite_out`: 1 unsigned int, both accumulated, so

@@ -487,14 +487,28 @@ def substep(scheme, U0, U1, params, dt_, step, bc):
     return d(np.min(c))
 
 
-def halo_copy_periodic_x(U, nx):
-    """One-rank east/west halo exchange with itself, MPISimulator.py:283-295, 440-480:
-    padded columns [nx, nx+2) -> [0, 2) and [2, 4) -> [nx+2, nx+4), all rows."""
+def halo_exchange_periodic(U, nx, ny, wrap_x, wrap_y):
+    """One-rank halo exchange with itself, MPISimulator.py:247-309, 371-480:
+    first north/south (rows [ny, ny+2) -> [0, 2) and [2, 4) -> [ny+2, ny+4),
+    interior columns only), then east/west over all ny+4 rows
+    (columns [nx, nx+2) -> [0, 2) and [2, 4) -> [nx+2, nx+4))."""
     for a in U:
-        west_src = a[:, nx:nx + 2].copy()
-        east_src = a[:, 2:4].copy()
-        a[:, nx + 2:nx + 4] = east_src
-        a[:, 0:2] = west_src
+        if wrap_y:
+            south_src = a[ny:ny + 2, 2:nx + 2].copy()
+            north_src = a[2:4, 2:nx + 2].copy()
+            a[0:2, 2:nx + 2] = south_src
+            a[ny + 2:ny + 4, 2:nx + 2] = north_src
+        if wrap_x:
+            west_src = a[:, nx:nx + 2].copy()
+            east_src = a[:, 2:4].copy()
+            a[:, nx + 2:nx + 4] = east_src
+            a[:, 0:2] = west_src
+
+
+def halo_copy_periodic_x(U, nx):
+    """East/west leg only (the case of a single row-strip rank with periodic x)."""
+    ny = U[0].shape[0] - 4
+    halo_exchange_periodic(U, nx, ny, True, False)
 
 
 def pad(field, dtype):
@@ -519,7 +533,7 @@ class OracleSimulator:
     """
 
     def __init__(self, scheme, fields, nx, ny, dx, dy, g, gamma=None, theta=None,
-                 cfl_scale=0.9, bc=None, dtype=np.float32, halo_periodic_x=False,
+                 cfl_scale=0.9, bc=None, dtype=np.float32, halo=None,
                  waf_static_dt=False):
         self.scheme = scheme
         self.dtype = np.dtype(dtype)
@@ -531,13 +545,15 @@ class OracleSimulator:
                        "gamma": d(gamma if gamma is not None else 0.0)}
         self.cfl_scale = cfl_scale
         self.bc = pack_bc(REFLECTIVE, REFLECTIVE, REFLECTIVE, REFLECTIVE) if bc is None else int(bc)
-        self.halo_periodic_x = halo_periodic_x
+        # (wrap_x, wrap_y): one-rank halo exchange after every substep, the
+        # MPISimulator path with the periodic edges of `bc` already set to Dirichlet
+        self.halo = halo
         self.waf_static_dt = waf_static_dt
         fields = [np.asarray(f) for f in fields]
         self.u0 = [pad(f, dtype) if f.shape == (self.ny, self.nx) else np.array(f, dtype=dtype) for f in fields]
         self.u1 = [np.zeros_like(a) for a in self.u0]
-        if halo_periodic_x:
-            halo_copy_periodic_x(self.u0, self.nx)
+        if halo:
+            halo_exchange_periodic(self.u0, self.nx, self.ny, *halo)
         # initial fill, KP07_dimsplit.py:103-106 (host NumPy on the caller's arrays)
         I = (slice(2, self.ny + 2), slice(2, self.nx + 2))
         q = [a[I] for a in self.u0]
@@ -556,8 +572,8 @@ class OracleSimulator:
         if not (self.scheme == "waf" and self.waf_static_dt):
             self.cfl = c
         self.u0, self.u1 = self.u1, self.u0
-        if self.halo_periodic_x:
-            halo_copy_periodic_x(self.u0, self.nx)
+        if self.halo:
+            halo_exchange_periodic(self.u0, self.nx, self.ny, *self.halo)
 
     def step(self, dt):
         d = self.dtype.type

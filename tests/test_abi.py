@@ -1,0 +1,101 @@
+"""C ABI of libfvmi.so (CPU only: no call reaches a GPU).
+
+* the library loads and exports every function include/fvmi.h declares;
+* the ctypes binding declares exactly those functions;
+* argument validation fails loudly, before any device work, with the
+  reference's error behaviour (Neumann boundaries rejected, Simulator.py:68-72).
+"""
+
+import ctypes
+import os
+import re
+
+import pytest
+
+from finitevolumegpu_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "fvmi.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(fvmi_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_entry_points():
+    names = header_functions()
+    for required in ("fvmi_swe_kp07_dimsplit_substep", "fvmi_swe_hll2_substep", "fvmi_swe_waf_substep",
+                     "fvmi_euler_kp07_dimsplit_substep", "fvmi_halo_copy_periodic_x", "fvmi_ctx_create",
+                     "fvmi_malloc_pitch", "fvmi_memcpy2d_async", "fvmi_last_error"):
+        assert required in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.lib()
+    missing = [n for n in header_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_matches_header():
+    assert sorted(_lib.SIGNATURES) == header_functions()
+
+
+def test_version_without_gpu():
+    assert _lib.lib().fvmi_version() == 1
+
+
+def _substep(nx=16, ny=16, bc=(3 << 24) | (3 << 16) | (3 << 8) | 3, step=0, dt=0.1, ptr=0x1000, pitch=256):
+    lib = _lib.lib()
+    return lib.fvmi_swe_kp07_dimsplit_substep(nx, ny, 1.0, 1.0, dt, 9.81, 1.3, step, bc,
+                                              ptr, pitch, ptr + 0x100000, pitch, ptr + 0x200000, pitch,
+                                              ptr + 0x300000, pitch, ptr + 0x400000, pitch, ptr + 0x500000, pitch,
+                                              None, None, None)
+
+
+def _err():
+    return _lib.lib().fvmi_last_error().decode()
+
+
+def test_validation_rejects_tiny_grid():
+    assert _substep(nx=1) == -1
+    assert "too small" in _err()
+
+
+def test_validation_rejects_neumann():
+    neumann = (1 << 24) | (3 << 16) | (3 << 8) | 3
+    assert _substep(bc=neumann) == -3
+    assert "Neumann" in _err()
+
+
+def test_validation_rejects_bad_step_dt_pitch():
+    assert _substep(step=2) == -1
+    assert _substep(dt=float("nan")) == -1
+    assert _substep(pitch=16) == -1
+    assert "pitch" in _err()
+
+
+def test_validation_rejects_aliasing():
+    lib = _lib.lib()
+    p = 0x1000
+    rc = lib.fvmi_swe_hll2_substep(8, 8, 1.0, 1.0, 0.1, 9.81, 1.8, 0, 0, p, 64, p + 4096, 64, p + 8192, 64,
+                                   p, 64, p + 4096 * 4, 64, p + 4096 * 5, 64, None, None, None)
+    assert rc == -1 and "alias" in _err()
+
+
+def test_validation_rejects_bad_subarea():
+    lib = _lib.lib()
+    p = 0x1000
+    args = [p + 0x10000 * k for k in range(8)]
+    rc = lib.fvmi_euler_kp07_dimsplit_substep(8, 8, 1.0, 1.0, 0.1, 0.1, 1.4, 1.3, 0, 0,
+                                              args[0], 64, args[1], 64, args[2], 64, args[3], 64,
+                                              args[4], 64, args[5], 64, args[6], 64, args[7], 64,
+                                              None, 4, 0, 2, 8, None, None)
+    assert rc == -1 and "sub-area" in _err()
+
+
+def test_missing_library_raises(monkeypatch, tmp_path):
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "absent.so"))
+    with pytest.raises(_lib.FvmiError):
+        _lib.lib()
