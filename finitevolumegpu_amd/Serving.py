@@ -1,0 +1,187 @@
+"""Continuous-batching simulation server (no counterpart in the reference).
+
+Requests (a scheme, initial fields, grid spacing, boundary conditions and a
+t_end) arrive over time.  The server keeps one ``Ensemble`` slot pool per
+scheme on its own stream, all state resident in device memory, and loops:
+
+1. admit every request that has arrived, while the device-memory budget
+   allows (two padded buffers per field); the rest wait in arrival order;
+2. enqueue one block of steps for every pool with work (pools run
+   concurrently on their streams, requests that joined at different times
+   share each launch);
+3. retire finished instances: download the result, record the completion
+   time, free the slot and its memory.
+
+Because an instance's arithmetic and dt policy do not depend on what else is
+in its pool, each result equals the plain Simulator.simulate() result of the
+same request alone (bitwise, same arithmetic policy).
+
+``run_sequential`` is the baseline: the same requests served one at a time
+through the plain Simulator API, in arrival order.
+"""
+
+import collections
+import logging
+import time
+
+import numpy as np
+
+from .Ensemble import Ensemble, _bc_code
+from .HipContext import Stream
+from .Simulator import BoundaryCondition
+
+
+Completed = collections.namedtuple("Completed", "id scheme arrival admitted finished steps cells fields")
+
+
+class Server:
+    def __init__(self, context, budget_bytes=200e9, steps_per_block=8, math="fast", capacity=4096,
+                 keep_results=True, use_graph=True):
+        self.logger = logging.getLogger(__name__)
+        self.context = context
+        self.budget = float(budget_bytes)
+        self.steps_per_block = int(steps_per_block)
+        self.math = math
+        self.capacity = int(capacity)
+        self.keep_results = keep_results
+        self.use_graph = use_graph
+        self.pools = {}
+        self.used_bytes = 0
+        self.owner = {}  # (scheme, slot) -> request record
+        self.completed = []
+        self.rejected = []
+
+    def _pool(self, scheme):
+        if scheme not in self.pools:
+            self.pools[scheme] = Ensemble(self.context, scheme, capacity=self.capacity, math=self.math,
+                                          steps_per_graph=self.steps_per_block, use_graph=self.use_graph,
+                                          stream=Stream())
+        return self.pools[scheme]
+
+    def fits(self, req):
+        pool = self._pool(req["scheme"])
+        need = pool.bytes_of(req)
+        if need > self.budget:
+            return None  # can never fit
+        return self.used_bytes + need <= self.budget and pool.free_slot() is not None
+
+    def admit(self, req, now):
+        pool = self._pool(req["scheme"])
+        slot = pool.add(req)
+        self.used_bytes += pool.bytes_of(req)
+        self.owner[(req["scheme"], slot)] = {"req": req, "admitted": now}
+        return slot
+
+    def _retire(self, now_fn):
+        done = 0
+        for scheme, pool in self.pools.items():
+            if not pool.live.any():
+                continue
+            for slot in pool.finished():
+                rec = self.owner.pop((scheme, slot))
+                req = rec["req"]
+                fields = pool.download(slot) if self.keep_results else None
+                t, nt = pool.state(slot)
+                now = now_fn()
+                self.completed.append(Completed(req["id"], scheme, req["arrival_time"], rec["admitted"], now, nt,
+                                                int(req["nx"]) * int(req["ny"]) * nt, fields))
+                self.used_bytes -= pool.bytes_of(req)
+                pool.remove(slot)
+                done += 1
+        return done
+
+    def serve(self, requests, realtime=True, max_wall=None):
+        """Serves ``requests`` (dicts as Ensemble instances plus ``id``, ``scheme``,
+        ``arrival`` seconds from the start).  With ``realtime`` the arrivals are
+        replayed against the wall clock; otherwise all requests are available at
+        once.  Returns a summary dict."""
+        pending = collections.deque(sorted(requests, key=lambda r: r["arrival"]))
+        waiting = collections.deque()
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0  # noqa: E731
+        for r in requests:
+            r["arrival_time"] = r["arrival"] if realtime else 0.0
+        while pending or waiting or self.owner:
+            # arrivals
+            while pending and (not realtime or pending[0]["arrival"] <= now()):
+                waiting.append(pending.popleft())
+            # admission in arrival order, against the memory budget
+            while waiting:
+                ok = self.fits(waiting[0])
+                if ok is None:
+                    self.rejected.append(waiting.popleft()["id"])
+                    continue
+                if not ok:
+                    break
+                self.admit(waiting.popleft(), now())
+            if not self.owner:
+                if pending and realtime:
+                    time.sleep(max(0.0, min(0.002, pending[0]["arrival"] - now())))
+                continue
+            # one block of steps on every pool, concurrently on their streams
+            for pool in self.pools.values():
+                if pool.live.any():
+                    pool.advance(wait=False)
+            for pool in self.pools.values():
+                if pool.live.any():
+                    pool.stream.synchronize()
+            self._retire(now)
+            if max_wall is not None and now() > max_wall:
+                break
+        wall = now()
+        return self.summary(wall)
+
+    def summary(self, wall):
+        lat = np.array([c.finished - c.arrival for c in self.completed])
+        cells = sum(c.cells for c in self.completed)
+        return {"requests": len(self.completed), "rejected": len(self.rejected), "wall_s": wall,
+                "cell_updates_per_s": cells / wall if wall > 0 else 0.0,
+                "p50_latency_s": float(np.percentile(lat, 50)) if lat.size else None,
+                "p99_latency_s": float(np.percentile(lat, 99)) if lat.size else None,
+                "cell_steps": cells}
+
+    def close(self):
+        for pool in self.pools.values():
+            pool.close()
+        self.pools = {}
+
+
+def make_simulator(context, req, math="fast"):
+    """Plain Simulator for one request (the baseline path)."""
+    from .EE2D_KP07_dimsplit import EE2D_KP07_dimsplit
+    from .HLL2 import HLL2
+    from .KP07_dimsplit import KP07_dimsplit
+    from .WAF import WAF
+    bc = BoundaryCondition(BoundaryCondition.getTypes(_bc_code(req.get("bc", BoundaryCondition()))))
+    f = req["fields"]
+    scheme = req["scheme"]
+    if scheme == "ee2d_kp07_dimsplit":
+        return EE2D_KP07_dimsplit(context, *f, req["nx"], req["ny"], req["dx"], req["dy"], req["g"],
+                                  req.get("gamma", 1.4), boundary_conditions=bc, math=math)
+    cls = {"kp07_dimsplit": KP07_dimsplit, "hll2": HLL2, "waf": WAF}[scheme]
+    return cls(context, *f, req["nx"], req["ny"], req["dx"], req["dy"], req["g"], boundary_conditions=bc, math=math)
+
+
+def run_sequential(context, requests, realtime=True, math="fast", keep_results=True):
+    """Baseline: one plain simulate() per request, in arrival order."""
+    t0 = time.perf_counter()
+    now = lambda: time.perf_counter() - t0  # noqa: E731
+    out = []
+    for req in sorted(requests, key=lambda r: r["arrival"]):
+        if realtime:
+            while now() < req["arrival"]:
+                time.sleep(min(0.002, req["arrival"] - now()))
+        sim = make_simulator(context, req, math)
+        sim.simulate(req["t_end"])
+        fields = sim.download() if keep_results else None
+        sim.synchronize()
+        arrival = req["arrival"] if realtime else 0.0
+        out.append(Completed(req["id"], req["scheme"], arrival, arrival, now(), sim.simSteps(),
+                             int(req["nx"]) * int(req["ny"]) * sim.simSteps(), fields))
+        del sim
+    wall = now()
+    lat = np.array([c.finished - c.arrival for c in out])
+    cells = sum(c.cells for c in out)
+    return out, {"requests": len(out), "wall_s": wall, "cell_updates_per_s": cells / wall,
+                 "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+                 "cell_steps": cells}

@@ -29,6 +29,19 @@ class LaunchConfig(ctypes.Structure):
     _fields_ = [("tile_rows", c_int), ("waves_per_block", c_int), ("math", c_int), ("xcd_remap", c_int)]
 
 
+class Instance(ctypes.Structure):
+    """fvmi_instance_t"""
+    _fields_ = [("field", (c_void_p * 4) * 2), ("pitch", c_int), ("nx", c_int), ("ny", c_int), ("bc", c_int),
+                ("dx", c_float), ("dy", c_float), ("g", c_float), ("theta", c_float),
+                ("gamma", c_float), ("sqrt_g", c_float), ("inv_g", c_float), ("inv_dx", c_float),
+                ("inv_dy", c_float), ("cfl", c_float * 2), ("dt_step", c_float),
+                ("nt", c_int), ("active", c_int), ("fixed_dt", c_int),
+                ("t", ctypes.c_double), ("t_end", ctypes.c_double), ("dt", ctypes.c_double),
+                ("cfl_scale", ctypes.c_double)]
+
+
+SCHEME_IDS = {"kp07_dimsplit": 0, "hll2": 1, "waf": 2, "ee2d_kp07_dimsplit": 3}
+
 _fp = P(c_float)
 
 # name -> (restype, argtypes)
@@ -73,6 +86,14 @@ SIGNATURES = {
     "fvmi_halo_copy_periodic_x": (c_int, [c_int, c_int, c_int, P(c_void_p), P(c_int), c_void_p]),
     "fvmi_halo_exchange_periodic": (c_int, [c_int, c_int, c_int, P(c_void_p), P(c_int), c_int, c_int, c_void_p]),
     "fvmi_field_stats": (c_int, [c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "fvmi_ensemble_prepare": (c_int, [c_void_p, c_int, c_void_p]),
+    "fvmi_ensemble_substep": (c_int, [c_int, c_void_p, c_void_p, c_int, c_int, P(LaunchConfig), c_void_p]),
+    "fvmi_ensemble_advance": (c_int, [c_void_p, c_int, c_void_p]),
+    "fvmi_ensemble_count_active": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
+    "fvmi_stream_begin_capture": (c_int, [c_void_p]),
+    "fvmi_stream_end_capture": (c_int, [c_void_p, P(c_void_p)]),
+    "fvmi_graph_launch": (c_int, [c_void_p, c_void_p]),
+    "fvmi_graph_destroy": (c_int, [c_void_p]),
 }
 
 _lib = None

@@ -1,0 +1,102 @@
+// Batched ensemble: many independent grids of one scheme advanced together.
+//
+// Device-resident instance table (fvmi_instance_t, include/fvmi.h) holds each
+// grid's fields, parameters and time-stepping state.  One full step of every
+// active instance is four launches, all reading the table on the device (so a
+// step sequence can be captured in a hipGraph and replayed with no host work):
+//
+//   prepare   per instance: active = t < t_end; every 100 steps dt = 0.5 *
+//             cfl_min * cfl_scale (double, as BaseSimulator.simulate computes
+//             it, Simulator.py:192-199); dt_step = float(min(dt, t_end - t));
+//             CFL slots reset
+//   substep 0 (x then y) and substep 1 (y then x): the single-grid march over a
+//             work list of segments (instance, strip, first row, rows) shared
+//             by all instances, so small and large grids fill the device
+//             together; inactive instances' segments return at once
+//   advance   per instance: t += dt_step (double), nt += 1
+//
+// Per-cell arithmetic is the single-grid kernel's, so an instance evolves
+// bitwise as the plain Simulator.simulate() loop would evolve it alone.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/fvmi.h"
+#include "fv_substep.h"
+
+namespace fvmi {
+
+struct EnsembleArgs {
+    fvmi_instance_t* table;
+    const int4* segs;  // (instance, strip_x, row_in, count)
+    int num_segs;
+    int step;
+    int waves_per_block;
+    int num_blocks;
+    int xcd_remap;
+};
+
+template <class S, class M>
+__global__ void __launch_bounds__(256) ensemble_substep_kernel(EnsembleArgs e) {
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    int lb = blockIdx.x;
+    if (e.xcd_remap) {
+        const int per = gridDim.x >> 3;
+        lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    }
+    const int gw = __builtin_amdgcn_readfirstlane(lb * e.waves_per_block + wave);
+    if (lb >= e.num_blocks || gw >= e.num_segs) return;
+    const int4 seg = e.segs[gw];
+    const int id = __builtin_amdgcn_readfirstlane(seg.x);
+    const fvmi_instance_t& I = e.table[id];
+    if (!I.active) return;
+
+    SubstepArgs a;
+    a.nx = I.nx;
+    a.ny = I.ny;
+    a.dx = I.dx;
+    a.dy = I.dy;
+    a.dt = I.dt_step * 0.5f;  // substep length, KP07_dimsplit.py:108-109
+    a.g = I.g;
+    a.theta = I.theta;
+    a.gamma = I.gamma;
+    a.step = e.step;
+    a.bc = I.bc;
+    a.x0 = 0;
+    a.y0 = 0;
+    a.x1 = I.nx;
+    a.y1 = I.ny;
+#pragma unroll
+    for (int k = 0; k < S::NV; ++k) {
+        // after a full step the state is back in buffer 0
+        a.in[k] = I.field[e.step][k];
+        a.out[k] = I.field[e.step ^ 1][k];
+        a.in_pitch[k] = I.pitch;
+        a.out_pitch[k] = I.pitch;
+    }
+    a.cfl = nullptr;
+    a.hg = 0.5f * I.g;
+    a.sqrt_g = I.sqrt_g;
+    a.inv_g = I.inv_g;
+    a.inv_dx = I.inv_dx;
+    a.inv_dy = I.inv_dy;
+    a.dtdx = a.dt / a.dx;
+    a.dtdy = a.dt / a.dy;
+
+    Strip st;
+    setup_strip<S, M>(a, st, lane, seg.y, seg.z, seg.w);
+    float acc;
+    if constexpr (S::kSlopes) {
+        acc = (e.step == 0) ? march_slopes<S, M, true>(a, st) : march_slopes<S, M, false>(a, st);
+    } else {
+        acc = (e.step == 0) ? march_waf<S, M, true>(a, st) : march_waf<S, M, false>(a, st);
+    }
+    float dt_min = acc;
+    if constexpr (!M::exact) dt_min = (acc > 0.0f) ? fast::rcp(acc) : __builtin_inff();
+    const float m = wave_min(dt_min);
+    if (lane == 0 && m < __builtin_inff())
+        atomicMin(reinterpret_cast<unsigned int*>(&e.table[id].cfl[e.step & 1]), __float_as_uint(m));
+}
+
+}  // namespace fvmi

@@ -7,4 +7,7 @@ namespace fvmi {
 hipError_t launch_substep_exact(int scheme, const SubstepArgs& a, int grid, hipStream_t stream) {
     return launch_substep<MathExact>(scheme, a, grid, stream);
 }
+hipError_t launch_ensemble_exact(int scheme, const EnsembleArgs& e, int grid, hipStream_t stream) {
+    return launch_ensemble_substep<MathExact>(scheme, e, grid, stream);
+}
 }  // namespace fvmi

@@ -6,4 +6,7 @@ namespace fvmi {
 hipError_t launch_substep_fast(int scheme, const SubstepArgs& a, int grid, hipStream_t stream) {
     return launch_substep<MathFast>(scheme, a, grid, stream);
 }
+hipError_t launch_ensemble_fast(int scheme, const EnsembleArgs& e, int grid, hipStream_t stream) {
+    return launch_ensemble_substep<MathFast>(scheme, e, grid, stream);
+}
 }  // namespace fvmi

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "fv_ensemble.h"
 #include "fv_substep.h"
 
 namespace fvmi {
@@ -53,7 +54,31 @@ hipError_t launch_substep(int scheme, SubstepArgs a, int grid, hipStream_t strea
     return hipGetLastError();
 }
 
+template <class M>
+hipError_t launch_ensemble_substep(int scheme, EnsembleArgs e, int grid, hipStream_t stream) {
+    const dim3 block(kLanes * e.waves_per_block);
+    switch (scheme) {
+        case SCHEME_KP07:
+            hipLaunchKernelGGL((ensemble_substep_kernel<SweKP07<M>, M>), dim3(grid), block, 0, stream, e);
+            break;
+        case SCHEME_HLL2:
+            hipLaunchKernelGGL((ensemble_substep_kernel<SweHLL2<M>, M>), dim3(grid), block, 0, stream, e);
+            break;
+        case SCHEME_WAF:
+            hipLaunchKernelGGL((ensemble_substep_kernel<SweWAF<M>, M>), dim3(grid), block, 0, stream, e);
+            break;
+        case SCHEME_EULER:
+            hipLaunchKernelGGL((ensemble_substep_kernel<EulerKP07<M>, M>), dim3(grid), block, 0, stream, e);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 // Defined in fv_kernels_exact.hip / fv_kernels_fast.hip.
+hipError_t launch_ensemble_exact(int scheme, const EnsembleArgs& e, int grid, hipStream_t stream);
+hipError_t launch_ensemble_fast(int scheme, const EnsembleArgs& e, int grid, hipStream_t stream);
 hipError_t launch_substep_exact(int scheme, const SubstepArgs& a, int grid, hipStream_t stream);
 hipError_t launch_substep_fast(int scheme, const SubstepArgs& a, int grid, hipStream_t stream);
 

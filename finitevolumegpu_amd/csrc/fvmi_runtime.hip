@@ -182,6 +182,52 @@ __global__ void halo_periodic_kernel(HaloArgs a) {
     reinterpret_cast<float*>(base + (size_t)r * a.pitch[var])[c] = v;
 }
 
+// Ensemble bookkeeping (one thread per instance); see fv_ensemble.h.
+__global__ void ensemble_prepare_kernel(fvmi_instance_t* table, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    fvmi_instance_t& I = table[i];
+    if (!I.active) return;
+    if (!(I.t < I.t_end)) {
+        I.active = 0;
+        return;
+    }
+    if (!I.fixed_dt && (I.nt % 100) == 0) I.dt = (double)I.cfl[1] * 0.5 * I.cfl_scale;
+    const double rem = I.t_end - I.t;
+    const float cur = (float)(I.dt < rem ? I.dt : rem);
+    if (!(cur > 0.0f)) {
+        I.active = 0;
+        return;
+    }
+    I.dt_step = cur;
+    I.cfl[0] = __builtin_inff();
+    I.cfl[1] = __builtin_inff();
+}
+
+__global__ void ensemble_advance_kernel(fvmi_instance_t* table, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    fvmi_instance_t& I = table[i];
+    if (I.active) {
+        I.t += (double)I.dt_step;
+        I.nt += 1;
+    }
+}
+
+__global__ void ensemble_count_kernel(const fvmi_instance_t* table, int n, int* out) {
+    __shared__ int partial[16];
+    int live = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) live += (table[i].active && table[i].t < table[i].t_end) ? 1 : 0;
+    for (int off = 32; off >= 1; off >>= 1) live += __shfl_xor(live, off, 64);
+    if ((threadIdx.x & 63) == 0) partial[threadIdx.x >> 6] = live;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int total = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) total += partial[w];
+        *out = total;
+    }
+}
+
 __global__ void field_stats_kernel(int nx, int ny, const float* f, int pitch, double* sum_out, unsigned int* bad_out) {
     double s = 0.0;
     unsigned int bad = 0;
@@ -476,6 +522,80 @@ int fvmi_halo_exchange_periodic(int nx, int ny, int nvars, float* const* fields,
 
 int fvmi_halo_copy_periodic_x(int nx, int ny, int nvars, float* const* fields, const int* pitches, void* stream) {
     return fvmi_halo_exchange_periodic(nx, ny, nvars, fields, pitches, 1, 0, stream);
+}
+
+int fvmi_ensemble_prepare(fvmi_instance_t* table, int n, void* stream) {
+    if (!table || n < 0) return fail(FVMI_ERR_INVALID, "bad ensemble table");
+    if (n == 0) return FVMI_OK;
+    hipLaunchKernelGGL(ensemble_prepare_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, table, n);
+    FVMI_HIP(hipGetLastError());
+    return FVMI_OK;
+}
+
+int fvmi_ensemble_advance(fvmi_instance_t* table, int n, void* stream) {
+    if (!table || n < 0) return fail(FVMI_ERR_INVALID, "bad ensemble table");
+    if (n == 0) return FVMI_OK;
+    hipLaunchKernelGGL(ensemble_advance_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, table, n);
+    FVMI_HIP(hipGetLastError());
+    return FVMI_OK;
+}
+
+int fvmi_ensemble_count_active(const fvmi_instance_t* table, int n, int* count_dev, void* stream) {
+    if (!table || n < 0 || !count_dev) return fail(FVMI_ERR_INVALID, "bad ensemble count arguments");
+    hipLaunchKernelGGL(ensemble_count_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, table, n, count_dev);
+    FVMI_HIP(hipGetLastError());
+    return FVMI_OK;
+}
+
+int fvmi_ensemble_substep(int scheme, fvmi_instance_t* table, const int* segs, int num_segs, int step,
+                          const fvmi_launch_t* launch, void* stream) {
+    if (!table || !segs || num_segs < 0) return fail(FVMI_ERR_INVALID, "bad ensemble work list");
+    if (scheme < SCHEME_KP07 || scheme > SCHEME_EULER) return fail(FVMI_ERR_INVALID, "unknown scheme %d", scheme);
+    if (step != 0 && step != 1) return fail(FVMI_ERR_INVALID, "step must be 0 or 1");
+    if (num_segs == 0) return FVMI_OK;
+    EnsembleArgs e{};
+    e.table = table;
+    e.segs = reinterpret_cast<const int4*>(segs);
+    e.num_segs = num_segs;
+    e.step = step;
+    e.waves_per_block = (launch && launch->waves_per_block > 0) ? launch->waves_per_block : 4;
+    if (e.waves_per_block > 4) return fail(FVMI_ERR_INVALID, "waves_per_block out of range [1, 4]");
+    e.xcd_remap = (launch && launch->xcd_remap == 2) ? 0 : 1;
+    const int math = launch ? launch->math : FVMI_MATH_FAST;
+    e.num_blocks = (num_segs + e.waves_per_block - 1) / e.waves_per_block;
+    int grid = e.num_blocks;
+    if (e.xcd_remap) grid = ((grid + 7) / 8) * 8;
+    hipError_t err = (math == FVMI_MATH_EXACT) ? launch_ensemble_exact(scheme, e, grid, (hipStream_t)stream)
+                                               : launch_ensemble_fast(scheme, e, grid, (hipStream_t)stream);
+    if (err != hipSuccess) return hip_fail(err, "ensemble substep launch");
+    return FVMI_OK;
+}
+
+int fvmi_stream_begin_capture(void* stream) {
+    FVMI_HIP(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
+    return FVMI_OK;
+}
+
+int fvmi_stream_end_capture(void* stream, void** graph_exec) {
+    if (!graph_exec) return fail(FVMI_ERR_INVALID, "graph_exec is NULL");
+    hipGraph_t g = nullptr;
+    FVMI_HIP(hipStreamEndCapture((hipStream_t)stream, &g));
+    hipGraphExec_t ge = nullptr;
+    hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+    *graph_exec = ge;
+    return FVMI_OK;
+}
+
+int fvmi_graph_launch(void* graph_exec, void* stream) {
+    FVMI_HIP(hipGraphLaunch((hipGraphExec_t)graph_exec, (hipStream_t)stream));
+    return FVMI_OK;
+}
+
+int fvmi_graph_destroy(void* graph_exec) {
+    if (graph_exec) FVMI_HIP(hipGraphExecDestroy((hipGraphExec_t)graph_exec));
+    return FVMI_OK;
 }
 
 int fvmi_field_stats(int nx, int ny, const float* field, int pitch, double* sum_out, unsigned int* nonfinite_out,

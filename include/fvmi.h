@@ -168,6 +168,57 @@ int fvmi_halo_exchange_periodic(int nx, int ny, int nvars, float* const* fields,
 int fvmi_field_stats(int nx, int ny, const float* field, int pitch, double* sum_out, unsigned int* nonfinite_out,
                      void* stream);
 
+/* ---- batched ensemble ------------------------------------------------------
+ * No counterpart in the reference, whose simulators run one grid per object
+ * (Simulator.py:109-266).  Many grids of one scheme are advanced together; an
+ * instance's time stepping reproduces BaseSimulator.simulate (Simulator.py:176-228)
+ * on the device, so each grid evolves exactly as it would alone.
+ *
+ * One full step of every instance = fvmi_ensemble_prepare, fvmi_ensemble_substep
+ * (step 0), fvmi_ensemble_substep (step 1), fvmi_ensemble_advance, all on one
+ * stream; the sequence may be captured in a graph (fvmi_stream_begin_capture).
+ */
+typedef struct fvmi_instance {
+    float* field[2][4]; /* [buffer][variable]: padded fields; buffer 0 holds the state between steps */
+    int pitch;          /* bytes, shared by all fields of the instance */
+    int nx, ny;
+    int bc;             /* packed boundary conditions */
+    float dx, dy, g, theta;
+    float gamma, sqrt_g, inv_g, inv_dx;
+    float inv_dy;
+    float cfl[2];       /* CFL minimum of substep 0 / 1; cfl[1] = initial CFL before the first step */
+    float dt_step;      /* float32 length of the current step */
+    int nt;             /* steps taken */
+    int active;         /* 0 once t >= t_end */
+    int fixed_dt;       /* 1: keep `dt`, no CFL refresh (simulate(t, dt=...)) */
+    double t, t_end, dt;
+    double cfl_scale;
+} fvmi_instance_t;
+
+/* dt policy and CFL reset for every instance (one thread each). */
+int fvmi_ensemble_prepare(fvmi_instance_t* table, int n, void* stream);
+/* One substep of every active instance over the work list `segs` (int4 per
+ * wavefront: instance, strip index, first interior row, row count; strips are
+ * 60 columns wide and a segment never crosses a strip). */
+int fvmi_ensemble_substep(int scheme, fvmi_instance_t* table, const int* segs, int num_segs, int step,
+                          const fvmi_launch_t* launch, void* stream);
+/* t += dt_step, nt += 1 for every instance that stepped. */
+int fvmi_ensemble_advance(fvmi_instance_t* table, int n, void* stream);
+/* Writes to *count_dev the number of instances that still have to step. */
+int fvmi_ensemble_count_active(const fvmi_instance_t* table, int n, int* count_dev, void* stream);
+
+#define FVMI_SCHEME_KP07_DIMSPLIT 0
+#define FVMI_SCHEME_HLL2 1
+#define FVMI_SCHEME_WAF 2
+#define FVMI_SCHEME_EE2D_KP07_DIMSPLIT 3
+
+/* ---- stream capture (hipGraph) -------------------------------------------- */
+int fvmi_stream_begin_capture(void* stream);
+/* Ends the capture and instantiates the graph. */
+int fvmi_stream_end_capture(void* stream, void** graph_exec);
+int fvmi_graph_launch(void* graph_exec, void* stream);
+int fvmi_graph_destroy(void* graph_exec);
+
 #ifdef __cplusplus
 }
 #endif
