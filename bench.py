@@ -2,7 +2,7 @@
 """Benchmark: KP07_dimsplit shallow water, 4096x4096 float32, on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--schemes kp07,hll2,waf,ee2d]
-                    [--no-cpu-baseline]
+                    [--no-cpu-baseline] [--ensemble 512] [--serve 2000 --serve-rate 200]
 
 A "step" is one full time step of the plain Simulator API (two fused substep
 launches: x-y then y-x sweeps), on the seeded dam break with 16 Gaussian bumps,
@@ -20,6 +20,11 @@ stream.  One JSON line is printed:
                     the same workload (512x512 crop of the generator, a few steps)
 * schemes           the same measurement for HLL2 and WAF at 4096^2 and
                     EE2D_KP07_dimsplit at 8192^2 (64 B per cell-step)
+* ensemble          512 dam breaks of uneven size in the batched engine against
+                    the plain simulate() loop over the same instances
+* serving           the continuous-batching server on 2000 Poisson-arriving
+                    requests of mixed scheme and size: p50/p99 latency and
+                    aggregate cell-updates/s against sequential plain solves
 """
 
 import argparse
@@ -50,6 +55,12 @@ def _parse():
     ap.add_argument("--waves-per-block", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy kernel 5 times (HBM counter calibration)")
+    ap.add_argument("--ensemble", type=int, default=512,
+                    help="instances of the batched-ensemble leg (0 = skip the leg)")
+    ap.add_argument("--serve", type=int, default=2000, help="requests of the serving leg (0 = skip the leg)")
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--sequential", type=int, default=-1,
+                    help="requests of the sequential serving baseline (-1 = all; the first K by arrival)")
     return ap.parse_args()
 
 
@@ -172,6 +183,74 @@ def _cpu_baseline(budget_s=12.0):
                       "(host os.cpu_count()=%s)" % (n, n, steps, el, os.cpu_count())}
 
 
+def _ensemble_leg(ctx, count):
+    """512 KP07_dimsplit dam breaks (side log-uniform in [64, 1024], per-instance
+    CFL dt to t_end U(5, 50)) in one batched engine, against the plain
+    Simulator.simulate() loop over the same instances.  Inputs are resident
+    before either clock starts; step counts must agree for every instance and
+    the fields bitwise for every 32nd instance."""
+    from finitevolumegpu_amd.Ensemble import Ensemble, _bc_code
+    from finitevolumegpu_amd.KP07_dimsplit import KP07_dimsplit
+    from finitevolumegpu_amd.Simulator import BoundaryCondition
+    from finitevolumegpu_amd.initial_conditions import ensemble_instances
+    insts = ensemble_instances(count, seed=0)
+    specs = [{"fields": [d["h"], d["hu"], d["hv"]], "nx": d["nx"], "ny": d["ny"], "dx": 1.0, "dy": 1.0,
+              "g": 9.81, "t_end": d["t_end"], "bc": d["bc"]} for d in insts]
+    ens = Ensemble(ctx, "kp07_dimsplit", specs, math="fast", steps_per_graph=8)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    blocks = ens.run()
+    ctx.synchronize()
+    wall_ens = time.perf_counter() - t0
+    cells_ens = ens.cell_steps()
+    nt_ens = ens.tab["nt"][:count].copy()
+    print("# ensemble: %d instances, %d blocks, %.3f s" % (count, blocks, wall_ens), file=sys.stderr, flush=True)
+    wall_plain, cells_plain, checked, equal, steps_equal = 0.0, 0, 0, 0, 0
+    for i, sp in enumerate(specs):
+        bc = BoundaryCondition(BoundaryCondition.getTypes(_bc_code(sp["bc"])))
+        sim = KP07_dimsplit(ctx, *sp["fields"], sp["nx"], sp["ny"], 1.0, 1.0, 9.81, boundary_conditions=bc,
+                            math="fast")
+        sim.synchronize()
+        t0 = time.perf_counter()
+        sim.simulate(sp["t_end"])
+        sim.synchronize()
+        wall_plain += time.perf_counter() - t0
+        cells_plain += sp["nx"] * sp["ny"] * sim.simSteps()
+        steps_equal += int(sim.simSteps() == nt_ens[i])
+        if i % 32 == 0:
+            checked += 1
+            equal += int(all(np.array_equal(a, b) for a, b in zip(sim.download(), ens.download(i))))
+        del sim
+    ens.close()
+    print("# plain loop: %.3f s" % wall_plain, file=sys.stderr, flush=True)
+    return {"workload": "%d KP07_dimsplit dam breaks, side log-U[64,1024], t_end U(5,50), per-instance CFL dt, "
+                        "reflective/outflow 50/50, fast arithmetic" % count,
+            "instances": count, "cell_steps": cells_ens, "wall_s": wall_ens,
+            "cell_updates_per_s": cells_ens / wall_ens, "plain_loop_wall_s": wall_plain,
+            "plain_loop_cell_updates_per_s": cells_plain / wall_plain, "speedup": wall_plain / wall_ens,
+            "step_counts_equal": "%d/%d" % (steps_equal, count), "bitwise_equal_sampled": "%d/%d" % (equal, checked)}
+
+
+def _serving_leg(ctx, count, rate, sequential):
+    """Continuous-batching server on Poisson arrivals (60 % KP07_dimsplit, 20 %
+    HLL2/WAF, 20 % EE2D, side log-uniform in [128, 2048], t_end U(2, 40),
+    200 GB budget) against sequential plain solves of the same requests."""
+    from finitevolumegpu_amd.Serving import Server, run_sequential
+    from finitevolumegpu_amd.initial_conditions import serving_requests
+    reqs = serving_requests(count, seed=0, rate=rate)
+    server = Server(ctx, budget_bytes=200e9, steps_per_block=8, math="fast", keep_results=False)
+    served = server.serve([dict(r) for r in reqs], realtime=True)
+    server.close()
+    print("# served: %s" % json.dumps(served), file=sys.stderr, flush=True)
+    k = count if sequential < 0 else min(count, sequential)
+    _, seq = run_sequential(ctx, [dict(r) for r in reqs[:k]], realtime=True, keep_results=False)
+    print("# sequential: %s" % json.dumps(seq), file=sys.stderr, flush=True)
+    return {"workload": "%d requests, Poisson %.0f/s, 60%% KP07 / 20%% HLL2|WAF / 20%% EE2D, side log-U[128,2048], "
+                        "t_end U(2,40), budget 200 GB, fast arithmetic" % (count, rate),
+            "served": served, "sequential": dict(seq, requests_of_total="%d/%d" % (k, count)),
+            "throughput_ratio": served["cell_updates_per_s"] / seq["cell_updates_per_s"]}
+
+
 def _traffic():
     path = os.path.join(ROOT, "profiles", "kp07_dimsplit_4096_hbm_traffic.json")
     if not os.path.exists(path):
@@ -223,6 +302,10 @@ def main():
         "cpu_baseline": cpu,
         "schemes": {k: {kk: v for kk, v in r.items() if kk != "launch"} for k, r in results.items()},
     }
+    if args.ensemble > 0:
+        line["ensemble"] = _ensemble_leg(ctx, args.ensemble)
+    if args.serve > 0:
+        line["serving"] = _serving_leg(ctx, args.serve, args.serve_rate, args.sequential)
     print(json.dumps(line), flush=True)
 
 

@@ -36,10 +36,36 @@ def _bc_code(bc):
     if isinstance(bc, BoundaryCondition):
         return int(bc.asCodedInt())
     if isinstance(bc, str):
-        kind = {"reflective": BoundaryCondition.Type.Reflective, "outflow": BoundaryCondition.Type.Outflow,
-                "periodic": BoundaryCondition.Type.Periodic}[bc]
+        T = BoundaryCondition.Type
+        if bc == "ns_reflective_ew_periodic":  # the Rayleigh-Taylor column
+            return int(BoundaryCondition({"north": T.Reflective, "south": T.Reflective, "east": T.Periodic,
+                                          "west": T.Periodic}).asCodedInt())
+        kind = {"reflective": T.Reflective, "outflow": T.Outflow, "periodic": T.Periodic}[bc]
         return int(BoundaryCondition.uniform(kind).asCodedInt())
     return int(bc)
+
+
+def plan_segments(ids, nx, ny, seg):
+    """Work list of the instances ``ids`` (sizes ``nx``, ``ny``): one int4
+    (instance, strip, first row, rows) per run of at most ``seg`` rows of a
+    60-column strip, every strip of every instance covered exactly once,
+    longest segments first (ties in instance, strip, row order)."""
+    ids = np.asarray(ids, dtype=np.int64)
+    sx = (np.asarray(nx, dtype=np.int64) + STRIP_COLS - 1) // STRIP_COLS
+    ny = np.asarray(ny, dtype=np.int64)
+    per_strip = (ny + seg - 1) // seg
+    count = sx * per_strip
+    if count.sum() == 0:
+        return np.zeros((0, 4), dtype=np.int32)
+    inst = np.repeat(np.arange(ids.size), count)
+    start = np.repeat(np.cumsum(count) - count, count)
+    j = np.arange(count.sum()) - start
+    strip = j // per_strip[inst]
+    r0 = (j % per_strip[inst]) * seg
+    rows = np.minimum(seg, ny[inst] - r0)
+    order = np.argsort(-rows, kind="stable")
+    out = np.stack([ids[inst], strip, r0, rows], axis=1)[order]
+    return np.ascontiguousarray(out, dtype=np.int32)
 
 
 class Ensemble:
@@ -73,6 +99,7 @@ class Ensemble:
         self.buffers = [None] * self.n
         self.live = np.zeros(self.n, dtype=bool)  # slots holding an instance
         self.host_table = (_lib.Instance * self.n)()
+        self.tab = np.frombuffer(self.host_table, dtype=_lib.INSTANCE_DTYPE)  # numpy view, same memory
         for i, spec in enumerate(instances):
             self._init_instance(i, spec)
             self.live[i] = True
@@ -85,6 +112,7 @@ class Ensemble:
         self.count_host = ctypes.c_void_p()
         _lib.call("fvmi_host_malloc", ctypes.byref(self.count_host), 4)
         self.segs_dev = None
+        self.segs_bytes = 0
         self.graph = None
         self.planned_active = None
         self.stream.synchronize()
@@ -148,10 +176,19 @@ class Ensemble:
                   ctypes.sizeof(self.host_table), _lib.MEMCPY_DTOH, self.stream.handle)
         self.stream.synchronize()
 
+    def _running(self):
+        """Live slots whose instance is still active and short of t_end (host copy)."""
+        return self.live & (self.tab["active"] != 0) & (self.tab["t"] < self.tab["t_end"])
+
     def _active_mask(self):
         self._download_table()
-        return np.array([bool(self.live[i]) and bool(self.host_table[i].active)
-                         and self.host_table[i].t < self.host_table[i].t_end for i in range(self.n)], dtype=bool)
+        return self._running()
+
+    def _strip_rows(self, mask):
+        """Rows of 60-column strips of the instances in ``mask`` (sum over them)."""
+        nx = self.tab["nx"][mask].astype(np.int64)
+        ny = self.tab["ny"][mask].astype(np.int64)
+        return int(((nx + STRIP_COLS - 1) // STRIP_COLS * ny).sum())
 
     # ------------------------------------------------------------------
     # slots
@@ -193,8 +230,7 @@ class Ensemble:
     def finished(self):
         """Live slots whose instance reached t_end."""
         self._download_table()
-        return [i for i in range(self.n) if self.live[i]
-                and not (self.host_table[i].active and self.host_table[i].t < self.host_table[i].t_end)]
+        return [int(i) for i in np.flatnonzero(self.live & ~self._running())]
 
     def bytes_of(self, spec):
         """Device memory an instance would take (two padded buffers per field)."""
@@ -205,31 +241,25 @@ class Ensemble:
     def plan(self, active):
         """Work list for the instances in ``active``: segments of at most seg_rows
         rows, never crossing a 60-column strip, longest first."""
-        rows = [(-(-int(self.host_table[i].nx) // STRIP_COLS), int(self.host_table[i].ny)) if self.live[i]
-                else (0, 0) for i in range(self.n)]
-        total = sum(sx * ny for i, (sx, ny) in enumerate(rows) if active[i])
+        active = np.asarray(active, dtype=bool) & self.live
+        total = self._strip_rows(active)
         if self.seg_rows:
             seg = int(self.seg_rows)
         else:
             slots = self.context.cu_count * 4 * self.waves_per_simd
             seg = max(self.min_seg_rows, -(-total // max(1, slots)))
-        segs = []
-        for i, (sx, ny) in enumerate(rows):
-            if not active[i]:
-                continue
-            for s in range(sx):
-                for r0 in range(0, ny, seg):
-                    segs.append((i, s, r0, min(seg, ny - r0)))
-        segs.sort(key=lambda t: -t[3])
-        arr = np.ascontiguousarray(np.array(segs, dtype=np.int32).reshape(-1, 4))
-        if self.segs_dev is not None:
-            _lib.call("fvmi_free", self.segs_dev)
-        self.segs_dev = ctypes.c_void_p()
-        _lib.call("fvmi_malloc", ctypes.byref(self.segs_dev), max(16, arr.nbytes))
+        arr = plan_segments(np.flatnonzero(active), self.tab["nx"][active], self.tab["ny"][active], seg)
+        if self.segs_dev is None or arr.nbytes > self.segs_bytes:
+            # grow only (a free would synchronise the device under other pools' work)
+            if self.segs_dev is not None:
+                _lib.call("fvmi_free", self.segs_dev)
+            self.segs_bytes = max(4096, 2 * arr.nbytes)
+            self.segs_dev = ctypes.c_void_p()
+            _lib.call("fvmi_malloc", ctypes.byref(self.segs_dev), self.segs_bytes)
         if arr.size:
             _lib.call("fvmi_memcpy_async", self.segs_dev, arr.ctypes.data, arr.nbytes, _lib.MEMCPY_HTOD,
                       self.stream.handle)
-        self.num_segs = len(segs)
+        self.num_segs = len(arr)
         self.planned_active = active.copy()
         self.planned_rows = total
         self.stream.synchronize()
@@ -280,9 +310,7 @@ class Ensemble:
         left = self.count_active()
         if left and left < self.planned_active.sum():
             active = self._active_mask()
-            rows = sum((-(-int(self.host_table[i].nx) // STRIP_COLS)) * int(self.host_table[i].ny)
-                       for i in range(self.n) if active[i])
-            if rows < replan_fraction * self.planned_rows:
+            if self._strip_rows(active) < replan_fraction * self.planned_rows:
                 self.plan(active)
         return left
 
@@ -306,8 +334,8 @@ class Ensemble:
     def cell_steps(self):
         """Sum over live instances of nx*ny*steps taken."""
         self._download_table()
-        return sum(int(self.host_table[i].nx) * int(self.host_table[i].ny) * int(self.host_table[i].nt)
-                   for i in range(self.n) if self.live[i])
+        m = self.live
+        return int((self.tab["nx"][m].astype(np.int64) * self.tab["ny"][m] * self.tab["nt"][m]).sum())
 
     def close(self):
         if self.graph is not None:

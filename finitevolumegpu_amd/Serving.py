@@ -21,6 +21,7 @@ through the plain Simulator API, in arrival order.
 """
 
 import collections
+import concurrent.futures
 import logging
 import time
 
@@ -34,9 +35,56 @@ from .Simulator import BoundaryCondition
 Completed = collections.namedtuple("Completed", "id scheme arrival admitted finished steps cells fields")
 
 
+def request_spec(req):
+    """The Ensemble instance dict of a request.  A request without ``fields``
+    (``initial_conditions.serving_requests``) is materialised here, at admission,
+    so the host never holds the initial data of requests still queued."""
+    if "fields" in req:
+        return req
+    from .initial_conditions import materialise
+    spec = dict(req)
+    spec.update(materialise(req))
+    spec.setdefault("nx", req["n"])
+    spec.setdefault("ny", req["n"])
+    return spec
+
+
+class _Prefetcher:
+    """Materialises requests on worker threads (NumPy releases the GIL), at
+    most ``depth`` ahead of the one being admitted."""
+
+    def __init__(self, workers=4, depth=8):
+        self.pool = concurrent.futures.ThreadPoolExecutor(max_workers=workers) if workers else None
+        self.depth = depth
+        self.futures = {}
+
+    def ahead(self, reqs):
+        """Starts materialising the first ``depth`` of ``reqs`` (arrived, in order)."""
+        if self.pool is None:
+            return
+        for k, req in enumerate(reqs):
+            if k >= self.depth:
+                break
+            if req["id"] not in self.futures:
+                self.futures[req["id"]] = self.pool.submit(request_spec, req)
+
+    def ready(self, req):
+        f = self.futures.get(req["id"])
+        return f is None or f.done()
+
+    def take(self, req):
+        f = self.futures.pop(req["id"], None)
+        return request_spec(req) if f is None else f.result()
+
+    def close(self):
+        if self.pool is not None:
+            self.pool.shutdown(wait=True, cancel_futures=True)
+        self.futures = {}
+
+
 class Server:
     def __init__(self, context, budget_bytes=200e9, steps_per_block=8, math="fast", capacity=4096,
-                 keep_results=True, use_graph=True):
+                 keep_results=True, use_graph=True, prefetch_workers=4):
         self.logger = logging.getLogger(__name__)
         self.context = context
         self.budget = float(budget_bytes)
@@ -45,6 +93,7 @@ class Server:
         self.capacity = int(capacity)
         self.keep_results = keep_results
         self.use_graph = use_graph
+        self.prefetch_workers = int(prefetch_workers)
         self.pools = {}
         self.used_bytes = 0
         self.owner = {}  # (scheme, slot) -> request record
@@ -65,9 +114,9 @@ class Server:
             return None  # can never fit
         return self.used_bytes + need <= self.budget and pool.free_slot() is not None
 
-    def admit(self, req, now):
+    def admit(self, req, now, spec=None):
         pool = self._pool(req["scheme"])
-        slot = pool.add(req)
+        slot = pool.add(spec if spec is not None else request_spec(req))
         self.used_bytes += pool.bytes_of(req)
         self.owner[(req["scheme"], slot)] = {"req": req, "admitted": now}
         return slot
@@ -97,37 +146,44 @@ class Server:
         once.  Returns a summary dict."""
         pending = collections.deque(sorted(requests, key=lambda r: r["arrival"]))
         waiting = collections.deque()
+        prefetch = _Prefetcher(self.prefetch_workers)
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0  # noqa: E731
         for r in requests:
             r["arrival_time"] = r["arrival"] if realtime else 0.0
-        while pending or waiting or self.owner:
-            # arrivals
-            while pending and (not realtime or pending[0]["arrival"] <= now()):
-                waiting.append(pending.popleft())
-            # admission in arrival order, against the memory budget
-            while waiting:
-                ok = self.fits(waiting[0])
-                if ok is None:
-                    self.rejected.append(waiting.popleft()["id"])
+        try:
+            while pending or waiting or self.owner:
+                # arrivals; their initial data are materialised on worker threads
+                while pending and (not realtime or pending[0]["arrival"] <= now()):
+                    waiting.append(pending.popleft())
+                prefetch.ahead(waiting)
+                # admission in arrival order, against the memory budget
+                while waiting:
+                    ok = self.fits(waiting[0])
+                    if ok is None:
+                        self.rejected.append(waiting.popleft()["id"])
+                        continue
+                    if not ok or (self.owner and not prefetch.ready(waiting[0])):
+                        break  # the device has work; admit it after this block
+                    req = waiting.popleft()
+                    self.admit(req, now(), prefetch.take(req))
+                    prefetch.ahead(waiting)
+                if not self.owner:
+                    if pending and realtime:
+                        time.sleep(max(0.0, min(0.002, pending[0]["arrival"] - now())))
                     continue
-                if not ok:
+                # one block of steps on every pool, concurrently on their streams
+                for pool in self.pools.values():
+                    if pool.live.any():
+                        pool.advance(wait=False)
+                for pool in self.pools.values():
+                    if pool.live.any():
+                        pool.stream.synchronize()
+                self._retire(now)
+                if max_wall is not None and now() > max_wall:
                     break
-                self.admit(waiting.popleft(), now())
-            if not self.owner:
-                if pending and realtime:
-                    time.sleep(max(0.0, min(0.002, pending[0]["arrival"] - now())))
-                continue
-            # one block of steps on every pool, concurrently on their streams
-            for pool in self.pools.values():
-                if pool.live.any():
-                    pool.advance(wait=False)
-            for pool in self.pools.values():
-                if pool.live.any():
-                    pool.stream.synchronize()
-            self._retire(now)
-            if max_wall is not None and now() > max_wall:
-                break
+        finally:
+            prefetch.close()
         wall = now()
         return self.summary(wall)
 
@@ -152,6 +208,7 @@ def make_simulator(context, req, math="fast"):
     from .HLL2 import HLL2
     from .KP07_dimsplit import KP07_dimsplit
     from .WAF import WAF
+    req = request_spec(req)
     bc = BoundaryCondition(BoundaryCondition.getTypes(_bc_code(req.get("bc", BoundaryCondition()))))
     f = req["fields"]
     scheme = req["scheme"]
@@ -162,16 +219,20 @@ def make_simulator(context, req, math="fast"):
     return cls(context, *f, req["nx"], req["ny"], req["dx"], req["dy"], req["g"], boundary_conditions=bc, math=math)
 
 
-def run_sequential(context, requests, realtime=True, math="fast", keep_results=True):
-    """Baseline: one plain simulate() per request, in arrival order."""
+def run_sequential(context, requests, realtime=True, math="fast", keep_results=True, prefetch_workers=4):
+    """Baseline: one plain simulate() per request, in arrival order (initial
+    data materialised ahead on worker threads, as the server does)."""
     t0 = time.perf_counter()
     now = lambda: time.perf_counter() - t0  # noqa: E731
     out = []
-    for req in sorted(requests, key=lambda r: r["arrival"]):
+    order = sorted(requests, key=lambda r: r["arrival"])
+    prefetch = _Prefetcher(prefetch_workers)
+    for k, req in enumerate(order):
         if realtime:
             while now() < req["arrival"]:
                 time.sleep(min(0.002, req["arrival"] - now()))
-        sim = make_simulator(context, req, math)
+        prefetch.ahead([r for r in order[k:k + prefetch.depth] if not realtime or r["arrival"] <= now()])
+        sim = make_simulator(context, prefetch.take(req), math)
         sim.simulate(req["t_end"])
         fields = sim.download() if keep_results else None
         sim.synchronize()
@@ -179,6 +240,7 @@ def run_sequential(context, requests, realtime=True, math="fast", keep_results=T
         out.append(Completed(req["id"], req["scheme"], arrival, arrival, now(), sim.simSteps(),
                              int(req["nx"]) * int(req["ny"]) * sim.simSteps(), fields))
         del sim
+    prefetch.close()
     wall = now()
     lat = np.array([c.finished - c.arrival for c in out])
     cells = sum(c.cells for c in out)

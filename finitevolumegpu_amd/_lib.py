@@ -7,6 +7,8 @@ There is no fallback: if the shared library is missing or cannot be loaded,
 import ctypes
 import os
 
+import numpy as np
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libfvmi.so")
 
@@ -39,6 +41,26 @@ class Instance(ctypes.Structure):
                 ("t", ctypes.c_double), ("t_end", ctypes.c_double), ("dt", ctypes.c_double),
                 ("cfl_scale", ctypes.c_double)]
 
+
+def _instance_dtype():
+    """numpy structured dtype with the offsets of ``Instance`` (a view of a host
+    table without per-field ctypes access)."""
+    kinds = {c_int: "<i4", c_float: "<f4", ctypes.c_double: "<f8"}
+    names, formats, offsets = [], [], []
+    for name, ctype in Instance._fields_:
+        if name == "field":
+            fmt = ("<u8", (2, 4))
+        elif name == "cfl":
+            fmt = ("<f4", (2,))
+        else:
+            fmt = kinds[ctype]
+        names.append(name)
+        formats.append(fmt)
+        offsets.append(getattr(Instance, name).offset)
+    return np.dtype({"names": names, "formats": formats, "offsets": offsets, "itemsize": ctypes.sizeof(Instance)})
+
+
+INSTANCE_DTYPE = _instance_dtype()
 
 SCHEME_IDS = {"kp07_dimsplit": 0, "hll2": 1, "waf": 2, "ee2d_kp07_dimsplit": 3}
 

@@ -92,11 +92,12 @@ __global__ void __launch_bounds__(256) ensemble_substep_kernel(EnsembleArgs e) {
     } else {
         acc = (e.step == 0) ? march_waf<S, M, true>(a, st) : march_waf<S, M, false>(a, st);
     }
+    if (e.step != 1) return;  // the CFL is taken after the y-then-x substep
     float dt_min = acc;
     if constexpr (!M::exact) dt_min = (acc > 0.0f) ? fast::rcp(acc) : __builtin_inff();
     const float m = wave_min(dt_min);
     if (lane == 0 && m < __builtin_inff())
-        atomicMin(reinterpret_cast<unsigned int*>(&e.table[id].cfl[e.step & 1]), __float_as_uint(m));
+        atomicMin(reinterpret_cast<unsigned int*>(&e.table[id].cfl[1]), __float_as_uint(m));
 }
 
 }  // namespace fvmi

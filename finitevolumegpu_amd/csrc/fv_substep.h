@@ -48,7 +48,7 @@ struct SubstepArgs {
     int in_pitch[4];  // bytes
     float* out[4];
     int out_pitch[4];
-    float* cfl;  // two-slot CFL minimum (slot = step&1 written, the other reset); may be null
+    float* cfl;  // two-slot CFL buffer: a step-1 launch minimises slot 1, every launch resets slot 1-step; may be null
     // Work decomposition: the sub-area is cut into 60-column strips; the
     // strips' rows, laid end to end (strip after strip), are split into equal
     // segments of `seg_rows` rows, one per wavefront, so every wave has the
@@ -389,7 +389,9 @@ __device__ __forceinline__ void slope_row(const SubstepArgs& a, const Strip& st,
         }
     }
     store_row<NV>(a, st, r, outq);
-    accumulate_cfl<S, M>(outq, a, st.lane_out, dt_min);
+    // the CFL minimum is taken after the y-then-x substep only: it is what
+    // computeDt reads after a full step (the last launch's values)
+    if constexpr (!XFirst) accumulate_cfl<S, M>(outq, a, st.lane_out, dt_min);
 }
 
 // WAF: the y window holds 4 rows and there is no slope state.
@@ -413,7 +415,9 @@ __device__ __forceinline__ void waf_row(const SubstepArgs& a, const Strip& st, c
     to_y<3>(y1, outq);
     if constexpr (!XFirst) x_sweep_row<S, M>(outq, a);
     store_row<3>(a, st, r, outq);
-    accumulate_cfl<S, M>(outq, a, st.lane_out, dt_min);
+    // the CFL minimum is taken after the y-then-x substep only: it is what
+    // computeDt reads after a full step (the last launch's values)
+    if constexpr (!XFirst) accumulate_cfl<S, M>(outq, a, st.lane_out, dt_min);
 }
 
 template <int NV>
@@ -569,10 +573,10 @@ __global__ void __launch_bounds__(256) substep_kernel(SubstepArgs a) {
     }
     float dt_min = acc;
     if constexpr (!M::exact) dt_min = (acc > 0.0f) ? fast::rcp(acc) : __builtin_inff();
-    if (a.cfl != nullptr) {
+    if (a.cfl != nullptr && a.step == 1) {
         const float m = wave_min(dt_min);
         if (lane == 0 && m < __builtin_inff())
-            atomicMin(reinterpret_cast<unsigned int*>(a.cfl + (a.step & 1)), __float_as_uint(m));
+            atomicMin(reinterpret_cast<unsigned int*>(a.cfl + 1), __float_as_uint(m));
     }
 }
 

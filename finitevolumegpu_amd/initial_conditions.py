@@ -24,6 +24,22 @@ def _centres(n, d=1.0):
     return (np.arange(n, dtype=np.float64) + 0.5) * d
 
 
+def _disk(n, cx, cy, r, inside, outside=1.0):
+    """n x n float32 field: ``inside`` at the cell centres with
+    (x-cx)^2 < r^2 - (y-cy)^2, ``outside`` elsewhere.  Each row's disk cells are
+    one contiguous run, so the field is written row by row (cheap for the large
+    serving requests)."""
+    x = _centres(n)
+    dx2 = (x - cx) ** 2
+    thr = r * r - (x - cy) ** 2
+    h = np.full((n, n), outside, dtype=np.float32)
+    for j in np.flatnonzero(thr > 0.0):
+        idx = np.flatnonzero(dx2 < thr[j])
+        if idx.size:
+            h[j, idx[0]:idx[-1] + 1] = inside
+    return h
+
+
 def circular_dam_break(nx=256, ny=256, radius=32.0, depth=1.0, jump=0.5,
                        noise=1e-3, seed=0, centre=None):
     """h = depth + jump*[r < radius] + U(-noise, noise); hu = hv = 0 (dx = dy = 1)."""
@@ -108,9 +124,7 @@ def ensemble_instances(count=512, seed=0, side_min=64, side_max=1024,
         cy = rng.uniform(r, s - r)
         bc = "reflective" if rng.uniform() < 0.5 else "outflow"
         t_end = rng.uniform(t_min, t_max)
-        x = _centres(s)
-        rr = np.hypot(x[None, :] - cx, x[:, None] - cy)
-        h = np.where(rr < r, ratio, 1.0).astype(np.float32)
+        h = _disk(s, cx, cy, r, ratio)
         z = np.zeros((s, s), dtype=np.float32)
         out.append({"nx": s, "ny": s, "h": h, "hu": z, "hv": z.copy(), "t_end": float(t_end), "bc": bc})
     return out
@@ -122,7 +136,7 @@ def serving_requests(count=2000, seed=0, rate=None, side_min=128, side_max=2048,
     60 % KP07_dimsplit, 20 % HLL2 or WAF, 20 % EE2D_KP07_dimsplit, side log-uniform in
     [side_min, side_max].  Initial data are generated lazily by ``materialise``.
 
-    Returns a list of dicts: arrival, scheme, n, t_end, seed."""
+    Returns a list of dicts: id, arrival, scheme, n (= nx = ny), t_end, seed."""
     rng = np.random.default_rng(seed)
     t = 0.0
     reqs = []
@@ -137,7 +151,7 @@ def serving_requests(count=2000, seed=0, rate=None, side_min=128, side_max=2048,
         else:
             scheme = "ee2d_kp07_dimsplit"
         n = int(round(math.exp(rng.uniform(math.log(side_min), math.log(side_max)))))
-        reqs.append({"id": i, "arrival": t, "scheme": scheme, "n": n,
+        reqs.append({"id": i, "arrival": t, "scheme": scheme, "n": n, "nx": n, "ny": n,
                      "t_end": float(rng.uniform(t_min, t_max)), "seed": int(rng.integers(0, 2**31 - 1))})
     return reqs
 
@@ -157,9 +171,7 @@ def materialise(req):
     r = frac * n
     cx = rng.uniform(r, n - r)
     cy = rng.uniform(r, n - r)
-    x = _centres(n)
-    rr = np.hypot(x[None, :] - cx, x[:, None] - cy)
-    h = np.where(rr < r, ratio, 1.0).astype(np.float32)
+    h = _disk(n, cx, cy, r, ratio)
     z = np.zeros((n, n), dtype=np.float32)
     return {"fields": [h, z, z.copy()], "dx": 1.0, "dy": 1.0, "g": 9.81, "gamma": None,
             "t_end": req["t_end"], "bc": "reflective"}

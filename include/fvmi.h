@@ -107,12 +107,13 @@ int fvmi_copy_f32(float* dst, const float* src, size_t count, void* stream);
  * fields (…1): step 0 sweeps x then y, step 1 y then x.  `dt` is the substep
  * length (the wrappers pass dt*0.5).  Boundary fill happens on the device.
  *
- * `cfl`, if not NULL, points to two device floats.  The kernel atomically
- * minimises slot (step & 1) with the per-cell admissible dt
- * min(dx/(|u|+c), dy/(|v|+c)) of every cell it writes and sets the other slot to
- * +inf, so after the usual step-0/step-1 pair slot 1 holds the CFL minimum of
- * the last substep (the reference's `gpuarray.min(cfl_data)`, KP07_dimsplit.py:135-137).
- * Before the first substep, set slot 0 to +inf and slot 1 to the initial value.
+ * `cfl`, if not NULL, points to two device floats.  A step-1 launch atomically
+ * minimises slot 1 with the per-cell admissible dt min(dx/(|u|+c), dy/(|v|+c))
+ * of every cell it writes; every launch sets the other slot (1 - step) to
+ * +inf.  After the usual step-0/step-1 pair slot 1 therefore holds the CFL
+ * minimum of the last substep, which is what the reference's
+ * `gpuarray.min(cfl_data)` reads after a full step (KP07_dimsplit.py:135-137).
+ * Before the first substep set slot 1 to the initial value.
  */
 
 /* KP07DimsplitKernel, SWE2D_KP07_dimsplit.cu:123-144 (argument order kept). */

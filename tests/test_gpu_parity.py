@@ -2,7 +2,8 @@
 against the float32 oracle on the same seeded inputs.
 
 * math='exact': bitwise equality of every field and of the CFL minimum.
-* math='fast' : relative L1 difference <= 1e-5 per field.
+* math='fast' : relative L1 difference <= 1e-5 per field (the Euler momenta of
+  a near-rest state: <= max(1e-5, 2 x the float32-vs-float64 oracle difference)).
 """
 
 import numpy as np
@@ -56,11 +57,34 @@ def test_exact_bitwise_small(ctx, scheme, bcname):
 @pytest.mark.parametrize("scheme", SCHEMES)
 @pytest.mark.parametrize("bcname", ["reflective", "mixed"])
 def test_fast_tolerance_small(ctx, scheme, bcname):
+    """Fast arithmetic within 1e-5 relative L1 of the float32 oracle per field.
+
+    The Euler state has momenta ~0.02 against a pressure ~2.5, so one rounding
+    of the pressure flux moves rho*u by ~1e-5 of its own size: for those two
+    fields the float32 oracle itself is only that close to the float64 oracle.
+    There the bar is max(1e-5, 2 x (float32 vs float64 oracle difference))."""
     print("case", scheme, bcname, flush=True)
     nx, ny = 131, 77
-    sim, ora = _run_pair(ctx, scheme, nx, ny, BCS[bcname], steps=20, math="fast")
-    for k, (a, b) in enumerate(zip(sim.download(), ora.download())):
-        assert O.rel_l1(a, b) <= FAST_TOL, (scheme, bcname, k, O.rel_l1(a, b))
+    steps = 20
+    sim, ora = _run_pair(ctx, scheme, nx, ny, BCS[bcname], steps=steps, math="fast")
+    got, ref = sim.download(), ora.download()
+    floor = [0.0] * len(ref)
+    if scheme == "ee2d_kp07_dimsplit":
+        fields = state(scheme, nx, ny, 3)
+        dx = dy = 1.0 / 64.0
+        ora64 = make_oracle(scheme, fields, nx, ny, dx, dy, BCS[bcname], dtype=np.float64)
+        for k, a in enumerate(ghost_frame(fields, len(fields))):
+            ora64.u0[k][:] = a
+            ora64.u1[k][:] = a
+        dt = np.float32(0.45 * float(make_oracle(scheme, fields, nx, ny, dx, dy, BCS[bcname]).cfl))
+        for _ in range(steps):
+            ora64.step(dt)
+        ref64 = ora64.download()
+        floor = [O.rel_l1(ref[k], ref64[k]) if k in (1, 2) else 0.0 for k in range(len(ref))]
+    for k, (a, b) in enumerate(zip(got, ref)):
+        err = O.rel_l1(a, b)
+        print("field", k, "fast-vs-f32", err, "f32-vs-f64", floor[k], flush=True)
+        assert err <= max(FAST_TOL, 2.0 * floor[k]), (scheme, bcname, k, err, floor[k])
 
 
 @pytest.mark.parametrize("tile_rows,waves", [(1, 1), (7, 3), (64, 4), (200, 2)])

@@ -47,8 +47,8 @@ def test_served_results_equal_plain_solves(ctx, math):
     peak = {"bytes": 0, "live": 0}
     orig = server.admit
 
-    def admit(req, now):
-        slot = orig(req, now)
+    def admit(req, now, spec=None):
+        slot = orig(req, now, spec)
         peak["bytes"] = max(peak["bytes"], server.used_bytes)
         peak["live"] = max(peak["live"], len(server.owner))
         return slot
@@ -82,8 +82,8 @@ def test_budget_admission_and_rejection(ctx):
     peak = [0]
     orig = server.admit
 
-    def admit(req, now):
-        slot = orig(req, now)
+    def admit(req, now, spec=None):
+        slot = orig(req, now, spec)
         peak[0] = max(peak[0], server.used_bytes)
         return slot
     server.admit = admit
@@ -92,4 +92,27 @@ def test_budget_admission_and_rejection(ctx):
     assert summary["requests"] == len(reqs)
     assert peak[0] <= budget
     assert server.used_bytes == 0
+    server.close()
+
+
+def test_generated_requests_served_equal_sequential(ctx):
+    """The generated serving workload (lazily materialised requests of all four
+    schemes, Poisson arrivals) served concurrently equals the sequential plain
+    solves of the same requests, bitwise."""
+    from finitevolumegpu_amd.Serving import Server, run_sequential
+    from finitevolumegpu_amd.initial_conditions import serving_requests
+    reqs = serving_requests(16, seed=3, rate=400.0, side_min=64, side_max=200, t_min=1.0, t_max=4.0)
+    assert {r["scheme"] for r in reqs} >= {"kp07_dimsplit", "ee2d_kp07_dimsplit"}
+    print("case generated serving", flush=True)
+    server = Server(ctx, steps_per_block=8, capacity=32)
+    summary = server.serve([dict(r) for r in reqs], realtime=True)
+    assert summary["requests"] == len(reqs) and summary["p99_latency_s"] >= summary["p50_latency_s"]
+    seq, seq_summary = run_sequential(ctx, [dict(r) for r in reqs], realtime=False)
+    by_id = {c.id: c for c in server.completed}
+    for c in seq:
+        got = by_id[c.id]
+        assert got.steps == c.steps and got.cells == c.cells
+        for k, (a, b) in enumerate(zip(got.fields, c.fields)):
+            assert np.array_equal(a, b), (c.id, c.scheme, k)
+    assert summary["cell_steps"] == seq_summary["cell_steps"]
     server.close()

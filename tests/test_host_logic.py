@@ -67,3 +67,43 @@ def test_simulate_fixed_dt():
     assert sim.dt_calls == []
     assert sim.nt == 4
     assert abs(sim.substeps[-1][0] - np.float32(0.1)) < 1e-7
+
+
+def test_ensemble_plan_segments_cover_each_row_once():
+    """The vectorised work list equals the plain nested loop (instance, strip,
+    row runs of `seg`, then a stable longest-first sort) and covers every row of
+    every 60-column strip exactly once."""
+    from finitevolumegpu_amd.Ensemble import plan_segments
+    rng = np.random.default_rng(4)
+    for seg in (1, 7, 24, 100):
+        ids = np.sort(rng.choice(50, size=13, replace=False))
+        nx = rng.integers(2, 300, size=ids.size)
+        ny = rng.integers(2, 300, size=ids.size)
+        got = plan_segments(ids, nx, ny, seg)
+        want = []
+        for i, a, b in zip(ids, nx, ny):
+            for s in range(-(-int(a) // 60)):
+                for r0 in range(0, int(b), seg):
+                    want.append((int(i), s, r0, min(seg, int(b) - r0)))
+        want.sort(key=lambda t: -t[3])
+        assert got.dtype == np.int32 and got.shape == (len(want), 4)
+        assert [tuple(r) for r in got.tolist()] == want
+        for i, a, b in zip(ids, nx, ny):
+            for s in range(-(-int(a) // 60)):
+                rows = np.zeros(int(b), dtype=int)
+                for _, _, r0, n in got[(got[:, 0] == i) & (got[:, 1] == s)]:
+                    rows[r0:r0 + n] += 1
+                assert (rows == 1).all()
+    assert plan_segments([], [], [], 24).shape == (0, 4)
+
+
+def test_instance_dtype_matches_ctypes_layout():
+    from finitevolumegpu_amd import _lib
+    t = (_lib.Instance * 3)()
+    view = np.frombuffer(t, dtype=_lib.INSTANCE_DTYPE)
+    t[1].nx, t[2].t, t[0].cfl[1], t[2].cfl_scale = 77, 5.5, 2.25, 0.9
+    t[1].field[1][3] = 0x1234
+    assert view["nx"].tolist() == [0, 77, 0] and view["t"][2] == 5.5 and view["cfl"][0, 1] == 2.25
+    assert view["cfl_scale"][2] == 0.9 and view["field"][1, 1, 3] == 0x1234
+    view["active"][2] = 1
+    assert t[2].active == 1
