@@ -189,6 +189,7 @@ def _ensemble_leg(ctx, count):
     Simulator.simulate() loop over the same instances.  Inputs are resident
     before either clock starts; step counts must agree for every instance and
     the fields bitwise for every 32nd instance."""
+    from finitevolumegpu_amd.Autotuner import Autotuner
     from finitevolumegpu_amd.Ensemble import Ensemble, _bc_code
     from finitevolumegpu_amd.KP07_dimsplit import KP07_dimsplit
     from finitevolumegpu_amd.Simulator import BoundaryCondition
@@ -196,7 +197,11 @@ def _ensemble_leg(ctx, count):
     insts = ensemble_instances(count, seed=0)
     specs = [{"fields": [d["h"], d["hu"], d["hv"]], "nx": d["nx"], "ny": d["ny"], "dx": 1.0, "dy": 1.0,
               "g": 9.81, "t_end": d["t_end"], "bc": d["bc"]} for d in insts]
-    ens = Ensemble(ctx, "kp07_dimsplit", specs, math="fast", steps_per_graph=8)
+    # schedule from the autotuner, timed on every 8th instance (untimed here)
+    sched = Autotuner.ensemble_schedule(ctx, "kp07_dimsplit", specs[::8])
+    print("# ensemble schedule: %s" % json.dumps(sched), file=sys.stderr, flush=True)
+    ens = Ensemble(ctx, "kp07_dimsplit", specs, math="fast", steps_per_graph=8, seg_rows=sched["seg_rows"],
+                   waves_per_block=sched["waves_per_block"])
     ctx.synchronize()
     t0 = time.perf_counter()
     blocks = ens.run()
@@ -217,6 +222,8 @@ def _ensemble_leg(ctx, count):
         wall_plain += time.perf_counter() - t0
         cells_plain += sp["nx"] * sp["ny"] * sim.simSteps()
         steps_equal += int(sim.simSteps() == nt_ens[i])
+        if (i + 1) % 64 == 0:
+            print("# plain loop: %d/%d instances, %.2f s" % (i + 1, count, wall_plain), file=sys.stderr, flush=True)
         if i % 32 == 0:
             checked += 1
             equal += int(all(np.array_equal(a, b) for a, b in zip(sim.download(), ens.download(i))))
@@ -225,7 +232,9 @@ def _ensemble_leg(ctx, count):
     print("# plain loop: %.3f s" % wall_plain, file=sys.stderr, flush=True)
     return {"workload": "%d KP07_dimsplit dam breaks, side log-U[64,1024], t_end U(5,50), per-instance CFL dt, "
                         "reflective/outflow 50/50, fast arithmetic" % count,
-            "instances": count, "cell_steps": cells_ens, "wall_s": wall_ens,
+            "instances": count, "schedule": {"seg_rows": sched["seg_rows"],
+                                             "waves_per_block": sched["waves_per_block"]},
+            "cell_steps": cells_ens, "wall_s": wall_ens,
             "cell_updates_per_s": cells_ens / wall_ens, "plain_loop_wall_s": wall_plain,
             "plain_loop_cell_updates_per_s": cells_plain / wall_plain, "speedup": wall_plain / wall_ens,
             "step_counts_equal": "%d/%d" % (steps_equal, count), "bitwise_equal_sampled": "%d/%d" % (equal, checked)}
@@ -238,12 +247,25 @@ def _serving_leg(ctx, count, rate, sequential):
     from finitevolumegpu_amd.Serving import Server, run_sequential
     from finitevolumegpu_amd.initial_conditions import serving_requests
     reqs = serving_requests(count, seed=0, rate=rate)
+
+    def progress(tag):
+        t0 = time.perf_counter()
+        mark = [0]
+
+        def report(done, total):
+            if done * 10 // total > mark[0] or done == total:
+                mark[0] = done * 10 // total
+                print("# %s: %d/%d requests done, %.1f s" % (tag, done, total, time.perf_counter() - t0),
+                      file=sys.stderr, flush=True)
+        return report
+
     server = Server(ctx, budget_bytes=200e9, steps_per_block=8, math="fast", keep_results=False)
-    served = server.serve([dict(r) for r in reqs], realtime=True)
+    served = server.serve([dict(r) for r in reqs], realtime=True, progress=progress("served"))
     server.close()
     print("# served: %s" % json.dumps(served), file=sys.stderr, flush=True)
     k = count if sequential < 0 else min(count, sequential)
-    _, seq = run_sequential(ctx, [dict(r) for r in reqs[:k]], realtime=True, keep_results=False)
+    _, seq = run_sequential(ctx, [dict(r) for r in reqs[:k]], realtime=True, keep_results=False,
+                            progress=progress("sequential"))
     print("# sequential: %s" % json.dumps(seq), file=sys.stderr, flush=True)
     return {"workload": "%d requests, Poisson %.0f/s, 60%% KP07 / 20%% HLL2|WAF / 20%% EE2D, side log-U[128,2048], "
                         "t_end U(2,40), budget 200 GB, fast arithmetic" % (count, rate),

@@ -11,7 +11,8 @@ rows per 64-lane strip (``tile_rows``) and strips per workgroup
 (``waves_per_block``).  Timing uses ``step()``.
 
 ``ensemble_schedule`` extends it to the batched engine: it times candidate
-tile heights for a list of grid sizes and returns the per-size choice.
+segment heights and workgroup sizes on the actual instance mix and returns
+the fastest (``tile_rows`` 0 and ``seg_rows`` None mean the balanced plan).
 """
 
 import logging
@@ -25,7 +26,7 @@ from .HipContext import Event, HipContext
 
 
 class Autotuner:
-    def __init__(self, nx=2048, ny=2048, tile_rows=(16, 32, 48, 64, 96, 128), waves_per_block=(1, 2, 4),
+    def __init__(self, nx=2048, ny=2048, tile_rows=(0, 16, 32, 48, 64, 96, 128), waves_per_block=(1, 2, 4),
                  filename=None, timesteps=10, warmup_timesteps=2):
         self.filename = filename or os.path.join(os.getcwd(), "autotuning_data_" + gethostname() + ".npz")
         self.nx = nx
@@ -151,6 +152,42 @@ class Autotuner:
         dt = 0.7 * min(dx, dy) / (0.1 * np.sqrt(2.0) + np.sqrt(g * 0.6))
         return {"h0": h, "hu0": hu, "hv0": hv, "nx": nx, "ny": ny, "dx": dx, "dy": dy, "g": g,
                 "dt": np.float32(dt), "bounds": [(0.3, 0.7), (-0.2, 0.2), (-0.2, 0.2)]}
+
+    @staticmethod
+    def ensemble_schedule(context, scheme, instances, seg_rows=(None, 24, 48, 96), waves_per_block=(2, 4),
+                          steps_per_graph=8, blocks=3, math="fast"):
+        """Schedule of the batched engine for ``instances`` (Ensemble dicts): for
+        every (segment rows, waves per workgroup) pair, builds the ensemble,
+        replays one graph block to warm up and times ``blocks`` more with events.
+        ``None`` segment rows is the balanced plan (all strip rows split over 4
+        waves per SIMD).  Returns {"seg_rows", "waves_per_block",
+        "cell_steps_per_s", "table"} for the fastest pair; the instances passed
+        in are not modified (each candidate runs on its own copy)."""
+        from .Ensemble import Ensemble
+        table = []
+        best = None
+        for sr in seg_rows:
+            for wpb in waves_per_block:
+                ens = Ensemble(context, scheme, instances, math=math, steps_per_graph=steps_per_graph,
+                               waves_per_block=wpb, seg_rows=sr)
+                try:
+                    ens.advance()
+                    c0 = ens.cell_steps()
+                    start, end = Event(), Event()
+                    start.record(ens.stream)
+                    for _ in range(blocks):
+                        ens.advance(wait=False)
+                    end.record(ens.stream)
+                    end.synchronize()
+                    seconds = end.time_since(start) * 1.0e-3
+                    rate = (ens.cell_steps() - c0) / seconds if seconds > 0 else 0.0
+                finally:
+                    ens.close()
+                table.append({"seg_rows": sr, "waves_per_block": wpb, "cell_steps_per_s": rate})
+                if best is None or rate > best["cell_steps_per_s"]:
+                    best = dict(table[-1])
+        best["table"] = table
+        return best
 
     @staticmethod
     def sanity_check(variable, bound_min, bound_max):

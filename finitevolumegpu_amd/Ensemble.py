@@ -9,7 +9,13 @@ a device table (``fvmi_instance_t``) and advances all instances together:
 * one work list of wavefront segments covers every active instance, balanced
   so that small and large grids share the device;
 * a full step is four launches (prepare, substep 0, substep 1, advance), and
-  ``steps_per_graph`` steps are captured once in a hipGraph and replayed;
+  ``steps_per_graph`` steps are captured once in a hipGraph and replayed; the
+  substep kernels read the work list and its length on the device, so the
+  graph survives instances joining and leaving (it is captured again only when
+  the list outgrows its buffer);
+* instance fields live in a ``DeviceArena`` (large device allocations carved
+  up on the host), so adding or retiring an instance never calls hipFree,
+  which would wait for the whole device;
 * between replays the host reads one integer (instances still running) and
   rebuilds the work list when finished instances leave too many idle segments.
 
@@ -19,13 +25,13 @@ with the single-grid kernel arithmetic, so its result is bitwise the result of
 the plain simulator run alone with the same arithmetic policy.
 """
 
+import bisect
 import ctypes
 import logging
 
 import numpy as np
 
 from . import _lib
-from .Common import HipArray2D
 from .HipContext import Stream
 from .Simulator import BoundaryCondition, swe_initial_cfl
 
@@ -68,6 +74,71 @@ def plan_segments(ids, nx, ny, seg):
     return np.ascontiguousarray(out, dtype=np.int32)
 
 
+class DeviceArena:
+    """First-fit sub-allocator over large device allocations (``chunk_bytes``
+    each, or one of exactly the size of a larger request).  ``alloc`` returns a
+    handle (chunk, offset, bytes); ``release`` is host bookkeeping only, with
+    neighbouring free blocks merged.  Chunks go back to the device in
+    ``close``."""
+
+    ALIGN = 256
+
+    def __init__(self, chunk_bytes=256 << 20):
+        self.chunk_bytes = int(chunk_bytes)
+        self.chunks = []  # [c_void_p, bytes]
+        self.free = []    # per chunk: [offset, bytes] blocks sorted by offset
+        self.used = 0
+
+    def alloc(self, nbytes):
+        n = -(-int(nbytes) // self.ALIGN) * self.ALIGN
+        for c, blocks in enumerate(self.free):
+            for j, (off, size) in enumerate(blocks):
+                if size >= n:
+                    if size == n:
+                        del blocks[j]
+                    else:
+                        blocks[j] = [off + n, size - n]
+                    self.used += n
+                    return (c, off, n)
+        size = max(self.chunk_bytes, n)
+        ptr = ctypes.c_void_p()
+        _lib.call("fvmi_malloc", ctypes.byref(ptr), size)
+        self.chunks.append([ptr, size])
+        self.free.append([[n, size - n]] if size > n else [])
+        self.used += n
+        return (len(self.chunks) - 1, 0, n)
+
+    def address(self, handle):
+        return self.chunks[handle[0]][0].value + handle[1]
+
+    def release(self, handle):
+        c, off, n = handle
+        blocks = self.free[c]
+        i = bisect.bisect_left(blocks, [off, 0])
+        blocks.insert(i, [off, n])
+        if i + 1 < len(blocks) and blocks[i][0] + blocks[i][1] == blocks[i + 1][0]:
+            blocks[i][1] += blocks[i + 1][1]
+            del blocks[i + 1]
+        if i > 0 and blocks[i - 1][0] + blocks[i - 1][1] == blocks[i][0]:
+            blocks[i - 1][1] += blocks[i][1]
+            del blocks[i]
+        self.used -= n
+
+    def reserved(self):
+        return sum(size for _, size in self.chunks)
+
+    def close(self):
+        for ptr, _ in self.chunks:
+            if ptr.value:
+                _lib.lib().fvmi_free(ptr)
+        self.chunks, self.free, self.used = [], [], 0
+
+
+def instance_pitch(nx):
+    """Row pitch in bytes of an instance's padded fields (2 ghost cells a side)."""
+    return ((int(nx) + 4) * 4 + 255) // 256 * 256
+
+
 class Ensemble:
     """``instances``: list of dicts with keys
     ``fields`` (list of (ny, nx) arrays), ``nx``, ``ny``, ``dx``, ``dy``, ``g``,
@@ -76,7 +147,8 @@ class Ensemble:
     (default 0.9) and ``dt`` (fixed step, as simulate(t, dt=...))."""
 
     def __init__(self, context, scheme, instances=(), math="fast", steps_per_graph=8, waves_per_block=4,
-                 seg_rows=None, min_seg_rows=24, waves_per_simd=4, use_graph=True, capacity=None, stream=None):
+                 seg_rows=None, min_seg_rows=24, waves_per_simd=4, use_graph=True, capacity=None, stream=None,
+                 arena=None):
         self.logger = logging.getLogger(__name__)
         if scheme not in _lib.SCHEME_IDS:
             raise ValueError("unknown scheme %r" % scheme)
@@ -94,9 +166,12 @@ class Ensemble:
         self.use_graph = use_graph
         context.make_current()
         self.stream = stream if stream is not None else Stream()
+        self.arena = arena if arena is not None else DeviceArena()
+        self.own_arena = arena is None
+        self._host_refs = []  # host arrays of copies not yet known to be complete
         instances = list(instances)
         self.n = max(len(instances), int(capacity or 0), 1)
-        self.buffers = [None] * self.n
+        self.buffers = [None] * self.n  # arena handle of each live slot
         self.live = np.zeros(self.n, dtype=bool)  # slots holding an instance
         self.host_table = (_lib.Instance * self.n)()
         self.tab = np.frombuffer(self.host_table, dtype=_lib.INSTANCE_DTYPE)  # numpy view, same memory
@@ -111,11 +186,16 @@ class Ensemble:
         _lib.call("fvmi_malloc", ctypes.byref(self.count_dev), 4)
         self.count_host = ctypes.c_void_p()
         _lib.call("fvmi_host_malloc", ctypes.byref(self.count_host), 4)
+        self.nsegs_dev = ctypes.c_void_p()
+        _lib.call("fvmi_malloc", ctypes.byref(self.nsegs_dev), 4)
         self.segs_dev = None
-        self.segs_bytes = 0
+        self.segs_cap = 0     # segments the device list can hold
+        self.segs_host = None  # pinned staging: count + list
+        self.num_segs = 0
         self.graph = None
+        self.captures = 0
         self.planned_active = None
-        self.stream.synchronize()
+        self._sync()
 
     # ------------------------------------------------------------------
     def _init_instance(self, i, spec):
@@ -126,16 +206,22 @@ class Ensemble:
         for f in fields:
             if f.shape != (ny, nx):
                 raise ValueError("instance %d: field shape %s != (%d, %d)" % (i, f.shape, ny, nx))
-        bufs = [[HipArray2D(self.stream, nx, ny, 2, 2, f) for f in fields],
-                [HipArray2D(self.stream, nx, ny, 2, 2, None) for _ in fields]]
-        pitch = bufs[0][0].pitch
-        if any(b.pitch != pitch for row in bufs for b in row):
-            raise RuntimeError("fields of one instance must share a pitch")
-        self.buffers[i] = bufs
+        # one arena block: [buffer][variable] padded fields of (ny+4) rows each,
+        # zeroed (ghost memory included) and the initial state copied in
+        pitch = instance_pitch(nx)
+        rows = ny + 4
+        handle = self.arena.alloc(2 * self.nvars * rows * pitch)
+        base = self.arena.address(handle)
+        self.buffers[i] = handle
+        _lib.call("fvmi_fill_f32", base, pitch, nx + 4, 2 * self.nvars * rows, 0.0, self.stream.handle)
         I = self.host_table[i]
         for b in range(2):
             for k in range(self.nvars):
-                I.field[b][k] = bufs[b][k].data_ptr
+                I.field[b][k] = base + (b * self.nvars + k) * rows * pitch
+        for k, f in enumerate(fields):
+            _lib.call("fvmi_memcpy2d_async", I.field[0][k] + 2 * pitch + 8, pitch, f.ctypes.data, nx * 4, nx * 4, ny,
+                      _lib.MEMCPY_HTOD, self.stream.handle)
+            self._host_refs.append(f)
         dx, dy = np.float32(spec["dx"]), np.float32(spec["dy"])
         g = np.float32(spec["g"])
         I.pitch, I.nx, I.ny = pitch, nx, ny
@@ -171,10 +257,14 @@ class Ensemble:
         I.dt_step = 0.0
 
     # ------------------------------------------------------------------
+    def _sync(self):
+        self.stream.synchronize()
+        self._host_refs = []
+
     def _download_table(self):
         _lib.call("fvmi_memcpy_async", ctypes.addressof(self.host_table), self.table,
                   ctypes.sizeof(self.host_table), _lib.MEMCPY_DTOH, self.stream.handle)
-        self.stream.synchronize()
+        self._sync()
 
     def _running(self):
         """Live slots whose instance is still active and short of t_end (host copy)."""
@@ -197,7 +287,9 @@ class Ensemble:
         return int(free[0]) if free.size else None
 
     def add(self, spec):
-        """Puts a new instance in a free slot (between steps) and returns the slot."""
+        """Puts a new instance in a free slot and returns the slot.  The copies
+        are queued on the ensemble's stream behind any block in flight, so an
+        instance can join between replays without waiting for the device."""
         i = self.free_slot()
         if i is None:
             raise RuntimeError("ensemble is full (%d slots)" % self.n)
@@ -206,22 +298,20 @@ class Ensemble:
         size = ctypes.sizeof(_lib.Instance)
         _lib.call("fvmi_memcpy_async", self.table.value + i * size, ctypes.addressof(self.host_table[i]), size,
                   _lib.MEMCPY_HTOD, self.stream.handle)
-        self.stream.synchronize()
         self.planned_active = None  # work list rebuilt at the next advance
         return i
 
     def remove(self, i):
-        """Frees slot i (its instance must have finished or be abandoned)."""
+        """Frees slot i (its instance must have finished or be abandoned).  Waits
+        for the ensemble's stream before its memory can be handed out again."""
         if not self.live[i]:
             return
         self.host_table[i].active = 0
         size = ctypes.sizeof(_lib.Instance)
         _lib.call("fvmi_memcpy_async", self.table.value + i * size, ctypes.addressof(self.host_table[i]), size,
                   _lib.MEMCPY_HTOD, self.stream.handle)
-        self.stream.synchronize()
-        for row in self.buffers[i]:
-            for b in row:
-                b.free()
+        self._sync()
+        self.arena.release(self.buffers[i])
         self.buffers[i] = None
         self.live[i] = False
         if self.planned_active is not None and self.planned_active[i]:
@@ -234,13 +324,13 @@ class Ensemble:
 
     def bytes_of(self, spec):
         """Device memory an instance would take (two padded buffers per field)."""
-        nx, ny = int(spec["nx"]), int(spec["ny"])
-        pitch = ((nx + 4) * 4 + 255) // 256 * 256
-        return 2 * self.nvars * (ny + 4) * pitch
+        return 2 * self.nvars * (int(spec["ny"]) + 4) * instance_pitch(spec["nx"])
 
     def plan(self, active):
         """Work list for the instances in ``active``: segments of at most seg_rows
-        rows, never crossing a 60-column strip, longest first."""
+        rows, never crossing a 60-column strip, longest first.  The list and its
+        length are written to the device on the ensemble's stream; the graph
+        is captured again only if the list outgrows its buffer."""
         active = np.asarray(active, dtype=bool) & self.live
         total = self._strip_rows(active)
         if self.seg_rows:
@@ -249,30 +339,43 @@ class Ensemble:
             slots = self.context.cu_count * 4 * self.waves_per_simd
             seg = max(self.min_seg_rows, -(-total // max(1, slots)))
         arr = plan_segments(np.flatnonzero(active), self.tab["nx"][active], self.tab["ny"][active], seg)
-        if self.segs_dev is None or arr.nbytes > self.segs_bytes:
-            # grow only (a free would synchronise the device under other pools' work)
-            if self.segs_dev is not None:
-                _lib.call("fvmi_free", self.segs_dev)
-            self.segs_bytes = max(4096, 2 * arr.nbytes)
+        need = len(arr)
+        self._sync()  # the pinned staging buffer may still feed the previous upload
+        if self.segs_dev is None or need > self.segs_cap:
+            for name in ("segs_dev",):
+                if getattr(self, name) is not None:
+                    _lib.call("fvmi_free", getattr(self, name))
+            if self.segs_host is not None:
+                _lib.call("fvmi_host_free", self.segs_host)
+            self.segs_cap = max(256, 2 * need)
             self.segs_dev = ctypes.c_void_p()
-            _lib.call("fvmi_malloc", ctypes.byref(self.segs_dev), self.segs_bytes)
-        if arr.size:
-            _lib.call("fvmi_memcpy_async", self.segs_dev, arr.ctypes.data, arr.nbytes, _lib.MEMCPY_HTOD,
+            _lib.call("fvmi_malloc", ctypes.byref(self.segs_dev), 16 * self.segs_cap)
+            self.segs_host = ctypes.c_void_p()
+            _lib.call("fvmi_host_malloc", ctypes.byref(self.segs_host), 16 * self.segs_cap + 16)
+            self._drop_graph()
+        staging = np.ctypeslib.as_array(ctypes.cast(self.segs_host, ctypes.POINTER(ctypes.c_int32)),
+                                        shape=(4 * self.segs_cap + 4,))
+        staging[0] = need
+        if need:
+            staging[4:4 + 4 * need] = arr.reshape(-1)
+            _lib.call("fvmi_memcpy_async", self.segs_dev, self.segs_host.value + 16, 16 * need, _lib.MEMCPY_HTOD,
                       self.stream.handle)
-        self.num_segs = len(arr)
+        _lib.call("fvmi_memcpy_async", self.nsegs_dev, self.segs_host, 4, _lib.MEMCPY_HTOD, self.stream.handle)
+        self.num_segs = need
         self.planned_active = active.copy()
         self.planned_rows = total
-        self.stream.synchronize()
-        if self.graph is not None:
-            _lib.call("fvmi_graph_destroy", self.graph)
-            self.graph = None
+
+    def _waves(self):
+        """Wavefronts of a substep launch: one per segment up to four per SIMD."""
+        return int(min(self.segs_cap, self.context.cu_count * 4 * self.waves_per_simd))
 
     def _enqueue_steps(self, k):
+        waves = self._waves()
         for _ in range(k):
             _lib.call("fvmi_ensemble_prepare", self.table, self.n, self.stream.handle)
             for step in (0, 1):
-                _lib.call("fvmi_ensemble_substep", self.scheme_id, self.table, self.segs_dev, self.num_segs, step,
-                          self.launch, self.stream.handle)
+                _lib.call("fvmi_ensemble_substep", self.scheme_id, self.table, self.segs_dev, self.nsegs_dev, waves,
+                          step, self.launch, self.stream.handle)
             _lib.call("fvmi_ensemble_advance", self.table, self.n, self.stream.handle)
 
     def _capture(self):
@@ -284,11 +387,17 @@ class Ensemble:
             rc = _lib.lib().fvmi_stream_end_capture(self.stream.handle, ctypes.byref(g))
         _lib.check(rc, "fvmi_stream_end_capture")
         self.graph = g
+        self.captures += 1
+
+    def _drop_graph(self):
+        if self.graph is not None:
+            _lib.call("fvmi_graph_destroy", self.graph)
+            self.graph = None
 
     def count_active(self):
         _lib.call("fvmi_ensemble_count_active", self.table, self.n, self.count_dev, self.stream.handle)
         _lib.call("fvmi_memcpy_async", self.count_host, self.count_dev, 4, _lib.MEMCPY_DTOH, self.stream.handle)
-        self.stream.synchronize()
+        self._sync()
         return ctypes.c_int.from_address(self.count_host.value).value
 
     def advance(self, replan_fraction=0.75, wait=True):
@@ -296,7 +405,9 @@ class Ensemble:
         instances still running afterwards (``wait=False``: only enqueues and
         returns None)."""
         if self.planned_active is None:
-            self.plan(self._active_mask())
+            # the host copy of the table can only be behind the device, so its
+            # running set contains every instance that still has to step
+            self.plan(self._running())
         if self.num_segs == 0:
             return self.count_active() if wait else None
         if self.use_graph:
@@ -329,7 +440,17 @@ class Ensemble:
         return self.host_table[i].t, self.host_table[i].nt
 
     def download(self, i):
-        return [b.download(self.stream) for b in self.buffers[i][0]]
+        """Interior of every field of instance i (float32 (ny, nx) arrays)."""
+        I = self.host_table[i]
+        nx, ny, pitch = int(I.nx), int(I.ny), int(I.pitch)
+        out = []
+        for k in range(self.nvars):
+            a = np.empty((ny, nx), dtype=np.float32)
+            _lib.call("fvmi_memcpy2d_async", a.ctypes.data, nx * 4, I.field[0][k] + 2 * pitch + 8, pitch, nx * 4, ny,
+                      _lib.MEMCPY_DTOH, self.stream.handle)
+            out.append(a)
+        self._sync()
+        return out
 
     def cell_steps(self):
         """Sum over live instances of nx*ny*steps taken."""
@@ -338,23 +459,30 @@ class Ensemble:
         return int((self.tab["nx"][m].astype(np.int64) * self.tab["ny"][m] * self.tab["nt"][m]).sum())
 
     def close(self):
-        if self.graph is not None:
+        if getattr(self, "graph", None) is not None:
             _lib.lib().fvmi_graph_destroy(self.graph)
             self.graph = None
-        for p in ("segs_dev", "table", "count_dev"):
+        if getattr(self, "stream", None) is not None:
+            self.stream.synchronize()
+        for p in ("segs_dev", "nsegs_dev", "table", "count_dev"):
             ptr = getattr(self, p, None)
             if ptr is not None and ptr.value:
                 _lib.lib().fvmi_free(ptr)
-                setattr(self, p, None)
-        if getattr(self, "count_host", None) is not None and self.count_host.value:
-            _lib.lib().fvmi_host_free(self.count_host)
-            self.count_host = None
-        for bufs in self.buffers:
-            if bufs is None:
-                continue
-            for row in bufs:
-                for b in row:
-                    b.free()
+            setattr(self, p, None)
+        for p in ("count_host", "segs_host"):
+            ptr = getattr(self, p, None)
+            if ptr is not None and ptr.value:
+                _lib.lib().fvmi_host_free(ptr)
+            setattr(self, p, None)
+        arena = getattr(self, "arena", None)
+        if arena is not None:
+            if self.own_arena:
+                arena.close()
+            else:
+                for handle in self.buffers:
+                    if handle is not None:
+                        arena.release(handle)
+        self.arena = None
         self.buffers = [None] * self.n
         self.live[:] = False
 

@@ -27,7 +27,7 @@ import time
 
 import numpy as np
 
-from .Ensemble import Ensemble, _bc_code
+from .Ensemble import DeviceArena, Ensemble, _bc_code
 from .HipContext import Stream
 from .Simulator import BoundaryCondition
 
@@ -83,8 +83,12 @@ class _Prefetcher:
 
 
 class Server:
+    """Continuous-batching server: one Ensemble slot pool per scheme, all pools
+    carving instance memory out of one DeviceArena, admission against
+    ``budget_bytes`` of instance memory."""
+
     def __init__(self, context, budget_bytes=200e9, steps_per_block=8, math="fast", capacity=4096,
-                 keep_results=True, use_graph=True, prefetch_workers=4):
+                 keep_results=True, use_graph=True, prefetch_workers=4, chunk_bytes=1 << 30):
         self.logger = logging.getLogger(__name__)
         self.context = context
         self.budget = float(budget_bytes)
@@ -94,17 +98,19 @@ class Server:
         self.keep_results = keep_results
         self.use_graph = use_graph
         self.prefetch_workers = int(prefetch_workers)
+        self.arena = DeviceArena(chunk_bytes=int(min(chunk_bytes, max(self.budget, 1 << 20))))
         self.pools = {}
         self.used_bytes = 0
         self.owner = {}  # (scheme, slot) -> request record
         self.completed = []
         self.rejected = []
+        self.timing = collections.Counter()
 
     def _pool(self, scheme):
         if scheme not in self.pools:
             self.pools[scheme] = Ensemble(self.context, scheme, capacity=self.capacity, math=self.math,
                                           steps_per_graph=self.steps_per_block, use_graph=self.use_graph,
-                                          stream=Stream())
+                                          stream=Stream(), arena=self.arena)
         return self.pools[scheme]
 
     def fits(self, req):
@@ -126,11 +132,11 @@ class Server:
         for scheme, pool in self.pools.items():
             if not pool.live.any():
                 continue
-            for slot in pool.finished():
+            for slot in pool.finished():  # downloads the pool's table once
                 rec = self.owner.pop((scheme, slot))
                 req = rec["req"]
                 fields = pool.download(slot) if self.keep_results else None
-                t, nt = pool.state(slot)
+                nt = int(pool.tab["nt"][slot])
                 now = now_fn()
                 self.completed.append(Completed(req["id"], scheme, req["arrival_time"], rec["admitted"], now, nt,
                                                 int(req["nx"]) * int(req["ny"]) * nt, fields))
@@ -139,47 +145,64 @@ class Server:
                 done += 1
         return done
 
-    def serve(self, requests, realtime=True, max_wall=None):
+    def _admit_waiting(self, waiting, prefetch, now):
+        """Admits waiting requests in arrival order while the budget and the
+        pools' slots allow; stops at a request whose data are not ready yet if
+        the device has other work."""
+        while waiting:
+            ok = self.fits(waiting[0])
+            if ok is None:
+                self.rejected.append(waiting.popleft()["id"])
+                continue
+            if not ok or (self.owner and not prefetch.ready(waiting[0])):
+                break
+            req = waiting.popleft()
+            self.admit(req, now(), prefetch.take(req))
+            prefetch.ahead(waiting)
+
+    def serve(self, requests, realtime=True, max_wall=None, progress=None):
         """Serves ``requests`` (dicts as Ensemble instances plus ``id``, ``scheme``,
-        ``arrival`` seconds from the start).  With ``realtime`` the arrivals are
-        replayed against the wall clock; otherwise all requests are available at
-        once.  Returns a summary dict."""
+        ``arrival`` seconds from the start; without ``fields`` they are
+        materialised at admission).  With ``realtime`` the arrivals are
+        replayed against the wall clock; otherwise all requests are available
+        at once.  Each iteration enqueues one block of steps on every pool,
+        admits arrivals while the device runs it (their copies queue behind
+        the block), waits, and retires finished requests.  ``progress(done,
+        total)`` is called after each retirement pass.  Returns a summary dict."""
         pending = collections.deque(sorted(requests, key=lambda r: r["arrival"]))
         waiting = collections.deque()
         prefetch = _Prefetcher(self.prefetch_workers)
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0  # noqa: E731
+        clock = time.perf_counter
         for r in requests:
             r["arrival_time"] = r["arrival"] if realtime else 0.0
         try:
             while pending or waiting or self.owner:
-                # arrivals; their initial data are materialised on worker threads
                 while pending and (not realtime or pending[0]["arrival"] <= now()):
                     waiting.append(pending.popleft())
                 prefetch.ahead(waiting)
-                # admission in arrival order, against the memory budget
-                while waiting:
-                    ok = self.fits(waiting[0])
-                    if ok is None:
-                        self.rejected.append(waiting.popleft()["id"])
-                        continue
-                    if not ok or (self.owner and not prefetch.ready(waiting[0])):
-                        break  # the device has work; admit it after this block
-                    req = waiting.popleft()
-                    self.admit(req, now(), prefetch.take(req))
-                    prefetch.ahead(waiting)
-                if not self.owner:
-                    if pending and realtime:
+                c = clock()
+                busy = [pool for pool in self.pools.values() if pool.live.any()]
+                for pool in busy:  # one block of steps per pool, concurrently on their streams
+                    pool.advance(wait=False)
+                self.timing["enqueue_s"] += clock() - c
+                c = clock()
+                self._admit_waiting(waiting, prefetch, now)  # overlaps the blocks just enqueued
+                self.timing["admit_s"] += clock() - c
+                if not busy:
+                    if not self.owner and pending and realtime:
                         time.sleep(max(0.0, min(0.002, pending[0]["arrival"] - now())))
                     continue
-                # one block of steps on every pool, concurrently on their streams
-                for pool in self.pools.values():
-                    if pool.live.any():
-                        pool.advance(wait=False)
-                for pool in self.pools.values():
-                    if pool.live.any():
-                        pool.stream.synchronize()
-                self._retire(now)
+                c = clock()
+                for pool in busy:
+                    pool.stream.synchronize()
+                self.timing["wait_s"] += clock() - c
+                c = clock()
+                if self._retire(now) and progress is not None:
+                    progress(len(self.completed), len(requests))
+                self.timing["retire_s"] += clock() - c
+                self.timing["iterations"] += 1
                 if max_wall is not None and now() > max_wall:
                     break
         finally:
@@ -190,16 +213,21 @@ class Server:
     def summary(self, wall):
         lat = np.array([c.finished - c.arrival for c in self.completed])
         cells = sum(c.cells for c in self.completed)
-        return {"requests": len(self.completed), "rejected": len(self.rejected), "wall_s": wall,
-                "cell_updates_per_s": cells / wall if wall > 0 else 0.0,
-                "p50_latency_s": float(np.percentile(lat, 50)) if lat.size else None,
-                "p99_latency_s": float(np.percentile(lat, 99)) if lat.size else None,
-                "cell_steps": cells}
+        out = {"requests": len(self.completed), "rejected": len(self.rejected), "wall_s": wall,
+               "cell_updates_per_s": cells / wall if wall > 0 else 0.0,
+               "p50_latency_s": float(np.percentile(lat, 50)) if lat.size else None,
+               "p99_latency_s": float(np.percentile(lat, 99)) if lat.size else None,
+               "cell_steps": cells}
+        out["host_time"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in self.timing.items()}
+        out["host_time"]["graph_captures"] = sum(p.captures for p in self.pools.values())
+        out["arena_reserved_bytes"] = self.arena.reserved()
+        return out
 
     def close(self):
         for pool in self.pools.values():
             pool.close()
         self.pools = {}
+        self.arena.close()
 
 
 def make_simulator(context, req, math="fast"):
@@ -219,7 +247,8 @@ def make_simulator(context, req, math="fast"):
     return cls(context, *f, req["nx"], req["ny"], req["dx"], req["dy"], req["g"], boundary_conditions=bc, math=math)
 
 
-def run_sequential(context, requests, realtime=True, math="fast", keep_results=True, prefetch_workers=4):
+def run_sequential(context, requests, realtime=True, math="fast", keep_results=True, prefetch_workers=4,
+                   progress=None):
     """Baseline: one plain simulate() per request, in arrival order (initial
     data materialised ahead on worker threads, as the server does)."""
     t0 = time.perf_counter()
@@ -232,14 +261,17 @@ def run_sequential(context, requests, realtime=True, math="fast", keep_results=T
             while now() < req["arrival"]:
                 time.sleep(min(0.002, req["arrival"] - now()))
         prefetch.ahead([r for r in order[k:k + prefetch.depth] if not realtime or r["arrival"] <= now()])
-        sim = make_simulator(context, prefetch.take(req), math)
-        sim.simulate(req["t_end"])
+        spec = prefetch.take(req)
+        sim = make_simulator(context, spec, math)
+        sim.simulate(spec["t_end"])  # the materialised t_end (scaled for Euler requests)
         fields = sim.download() if keep_results else None
         sim.synchronize()
         arrival = req["arrival"] if realtime else 0.0
         out.append(Completed(req["id"], req["scheme"], arrival, arrival, now(), sim.simSteps(),
                              int(req["nx"]) * int(req["ny"]) * sim.simSteps(), fields))
         del sim
+        if progress is not None:
+            progress(len(out), len(order))
     prefetch.close()
     wall = now()
     lat = np.array([c.finished - c.arrival for c in out])

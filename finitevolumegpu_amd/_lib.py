@@ -109,7 +109,7 @@ SIGNATURES = {
     "fvmi_halo_exchange_periodic": (c_int, [c_int, c_int, c_int, P(c_void_p), P(c_int), c_int, c_int, c_void_p]),
     "fvmi_field_stats": (c_int, [c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "fvmi_ensemble_prepare": (c_int, [c_void_p, c_int, c_void_p]),
-    "fvmi_ensemble_substep": (c_int, [c_int, c_void_p, c_void_p, c_int, c_int, P(LaunchConfig), c_void_p]),
+    "fvmi_ensemble_substep": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_int, P(LaunchConfig), c_void_p]),
     "fvmi_ensemble_advance": (c_int, [c_void_p, c_int, c_void_p]),
     "fvmi_ensemble_count_active": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     "fvmi_stream_begin_capture": (c_int, [c_void_p]),

@@ -1,6 +1,6 @@
 """Builds libfvmi.so (gfx950) in-tree with hipcc.
 
-    python -m finitevolumegpu_amd.build [--verbose]
+    python -m finitevolumegpu_amd.build [--verbose] [--define NAME=VALUE ...] [--out PATH]
 
 Three translation units: the exact-arithmetic kernels (-ffp-contract=off, IEEE
 division/square root), the fast kernels (contraction allowed) and the C-ABI
@@ -37,25 +37,32 @@ def hipcc():
     return exe
 
 
-def build(verbose=False):
-    objdir = os.path.join(HERE, "_build")
+def build(verbose=False, defines=(), out=LIB):
+    """Compiles the three units and links ``out``.  ``defines``: extra
+    ``NAME=VALUE`` macros (e.g. FVMI_ROW_PREFETCH=2) for launch-shape studies;
+    a build with defines goes to its own object folder."""
+    tag = "_".join(d.replace("=", "") for d in defines)
+    objdir = os.path.join(HERE, "_build" + ("_" + tag if tag else ""))
     os.makedirs(objdir, exist_ok=True)
     objs = []
     for src, extra in UNITS:
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
-        cmd = [hipcc()] + COMMON + extra + ["-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [hipcc()] + COMMON + extra + ["-D" + d for d in defines] + ["-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
         objs.append(obj)
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":
-    print(build(verbose="--verbose" in sys.argv))
+    args = sys.argv[1:]
+    defs = [args[i + 1] for i, x in enumerate(args) if x == "--define"]
+    outs = [args[i + 1] for i, x in enumerate(args) if x == "--out"]
+    print(build(verbose="--verbose" in args, defines=defs, out=os.path.abspath(outs[0]) if outs else LIB))

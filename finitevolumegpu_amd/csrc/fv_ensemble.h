@@ -12,7 +12,9 @@
 //   substep 0 (x then y) and substep 1 (y then x): the single-grid march over a
 //             work list of segments (instance, strip, first row, rows) shared
 //             by all instances, so small and large grids fill the device
-//             together; inactive instances' segments return at once
+//             together; inactive instances' segments return at once.  The
+//             list's length is read on the device, so re-planning between
+//             graph replays needs no new graph
 //   advance   per instance: t += dt_step (double), nt += 1
 //
 // Per-cell arithmetic is the single-grid kernel's, so an instance evolves
@@ -28,26 +30,18 @@ namespace fvmi {
 
 struct EnsembleArgs {
     fvmi_instance_t* table;
-    const int4* segs;  // (instance, strip_x, row_in, count)
-    int num_segs;
+    const int4* segs;     // (instance, strip_x, row_in, count)
+    const int* num_segs;  // device: valid entries of segs
     int step;
     int waves_per_block;
     int num_blocks;
     int xcd_remap;
 };
 
+// One segment of one instance: the single-grid march with the instance's
+// parameters; returns after the CFL update (step 1).
 template <class S, class M>
-__global__ void __launch_bounds__(256) ensemble_substep_kernel(EnsembleArgs e) {
-    const int lane = threadIdx.x & (kLanes - 1);
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    int lb = blockIdx.x;
-    if (e.xcd_remap) {
-        const int per = gridDim.x >> 3;
-        lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    }
-    const int gw = __builtin_amdgcn_readfirstlane(lb * e.waves_per_block + wave);
-    if (lb >= e.num_blocks || gw >= e.num_segs) return;
-    const int4 seg = e.segs[gw];
+__device__ __forceinline__ void ensemble_segment(const EnsembleArgs& e, const int4 seg, int lane) {
     const int id = __builtin_amdgcn_readfirstlane(seg.x);
     const fvmi_instance_t& I = e.table[id];
     if (!I.active) return;
@@ -85,7 +79,8 @@ __global__ void __launch_bounds__(256) ensemble_substep_kernel(EnsembleArgs e) {
     a.dtdy = a.dt / a.dy;
 
     Strip st;
-    setup_strip<S, M>(a, st, lane, seg.y, seg.z, seg.w);
+    setup_strip<S, M>(a, st, lane, __builtin_amdgcn_readfirstlane(seg.y), __builtin_amdgcn_readfirstlane(seg.z),
+                      __builtin_amdgcn_readfirstlane(seg.w));
     float acc;
     if constexpr (S::kSlopes) {
         acc = (e.step == 0) ? march_slopes<S, M, true>(a, st) : march_slopes<S, M, false>(a, st);
@@ -98,6 +93,25 @@ __global__ void __launch_bounds__(256) ensemble_substep_kernel(EnsembleArgs e) {
     const float m = wave_min(dt_min);
     if (lane == 0 && m < __builtin_inff())
         atomicMin(reinterpret_cast<unsigned int*>(&e.table[id].cfl[1]), __float_as_uint(m));
+}
+
+// A fixed grid of waves walks the work list (wave w takes segments w, w + W,
+// ...; the list is sorted longest first), so the launch shape does not depend
+// on the list and a captured graph survives re-planning.
+template <class S, class M>
+__global__ void __launch_bounds__(256) ensemble_substep_kernel(EnsembleArgs e) {
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    int lb = blockIdx.x;
+    if (e.xcd_remap) {
+        const int per = gridDim.x >> 3;
+        lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    }
+    if (lb >= e.num_blocks) return;
+    const int total = e.num_blocks * e.waves_per_block;
+    const int n = __builtin_amdgcn_readfirstlane(*e.num_segs);
+    for (int s = __builtin_amdgcn_readfirstlane(lb * e.waves_per_block + wave); s < n; s += total)
+        ensemble_segment<S, M>(e, e.segs[s], lane);
 }
 
 }  // namespace fvmi

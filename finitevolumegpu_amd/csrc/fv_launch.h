@@ -19,13 +19,17 @@ inline int plan_substep(SubstepArgs& a, int cu_count, int waves_per_simd = 4, in
     const int wx = a.x1 - a.x0;
     a.rows_per_strip = a.y1 - a.y0;
     a.strips_x = (wx + kStripCols - 1) / kStripCols;
-    a.total_rows = a.strips_x * a.rows_per_strip;
     if (a.seg_rows <= 0) {
+        // about waves_per_simd waves on every SIMD: segments per strip rounded
+        // to the nearest count, then equal segment heights of at least min_seg
         const int target = cu_count * 4 * waves_per_simd;
-        int seg = (a.total_rows + target - 1) / target;
+        int k = (target + a.strips_x / 2) / a.strips_x;
+        if (k < 1) k = 1;
+        int seg = (a.rows_per_strip + k - 1) / k;
         a.seg_rows = seg < min_seg ? min_seg : seg;
     }
-    a.num_waves = (a.total_rows + a.seg_rows - 1) / a.seg_rows;
+    a.segs_per_strip = (a.rows_per_strip + a.seg_rows - 1) / a.seg_rows;
+    a.num_waves = a.strips_x * a.segs_per_strip;
     a.num_blocks = (a.num_waves + a.waves_per_block - 1) / a.waves_per_block;
     int grid = a.num_blocks;
     if (a.xcd_remap) grid = ((grid + 7) / 8) * 8;

@@ -49,15 +49,16 @@ struct SubstepArgs {
     float* out[4];
     int out_pitch[4];
     float* cfl;  // two-slot CFL buffer: a step-1 launch minimises slot 1, every launch resets slot 1-step; may be null
-    // Work decomposition: the sub-area is cut into 60-column strips; the
-    // strips' rows, laid end to end (strip after strip), are split into equal
-    // segments of `seg_rows` rows, one per wavefront, so every wave has the
-    // same amount of work (a segment may continue into the next strip).
+    // Work decomposition: the sub-area is cut into 60-column strips and every
+    // strip into `segs_per_strip` segments of `seg_rows` rows (the last one
+    // shorter), one per wavefront.  Segment j starts at the same row in every
+    // strip, so the waves of neighbouring strips (on the same XCD) read the
+    // cache lines they share at about the same time.
     int seg_rows;
+    int segs_per_strip;
     int strips_x;
     int rows_per_strip;  // y1 - y0
-    int total_rows;      // strips_x * rows_per_strip
-    int num_waves;
+    int num_waves;       // strips_x * segs_per_strip
     int waves_per_block;
     int num_blocks;  // logical blocks = ceil(num_waves / waves_per_block)
     int xcd_remap;
@@ -354,51 +355,66 @@ __device__ __forceinline__ void accumulate_cfl(const float (&q)[S::NV], const Su
     }
 }
 
-// Loop-carried state of the y sweep for slope schemes: the predicted top face
-// of the previous row and the flux through its top interface.
+// Rows loaded ahead of the row being processed.  Each wave keeps a ring of
+// row slots in registers: the rows its y stencil still needs plus this many
+// rows in flight, so a load is issued kRowPrefetch rows of work before its
+// first use.
+#ifndef FVMI_ROW_PREFETCH
+#define FVMI_ROW_PREFETCH 1
+#endif
+constexpr int kRowPrefetch = FVMI_ROW_PREFETCH;
+static_assert(kRowPrefetch >= 1 && kRowPrefetch <= 3, "row prefetch distance 1..3");
+
+// One row step of the y sweep for slope schemes.  Loop-carried state: qbt, the
+// predicted top face of row r-2, and g, the flux through its bottom interface;
+// both are replaced by the values of row r-1 (in place: reads precede writes).
 template <class S, class M, bool XFirst>
 __device__ __forceinline__ void slope_row(const SubstepArgs& a, const Strip& st, float kky,
                                           const float (&w0)[S::NV], const float (&w1)[S::NV], float (&w2)[S::NV],
-                                          const float (&qbt_in)[S::NV], float (&qbt_out)[S::NV],
-                                          const float (&g_in)[S::NV], float (&g_out)[S::NV], int r, bool emit,
+                                          float (&qbt)[S::NV], float (&g)[S::NV], int r, bool emit,
                                           float& dt_min) {
     // w0, w1: rows r-2, r-1 (already x swept when XFirst); w2: raw row r on entry.
     constexpr int NV = S::NV;
     if constexpr (XFirst) x_sweep_row<S, M>(w2, a);
-    float y0[NV], y1[NV], y2[NV], qbb[NV];
+    float y0[NV], y1[NV], y2[NV], qbb[NV], qbt_next[NV];
     to_y<NV>(w0, y0);
     to_y<NV>(w1, y1);
     to_y<NV>(w2, y2);
-    reconstruct<S, M>(y0, y1, y2, a, kky, qbb, qbt_out);  // faces of row r-1
+    reconstruct<S, M>(y0, y1, y2, a, kky, qbb, qbt_next);  // faces of row r-1
     float G[NV];
-    S::flux_y(qbt_in, qbb, a, G);  // interface (r-2, r-1)
+    S::flux_y(qbt, qbb, a, G);  // interface (r-2, r-1)
+    if (emit) {
+        float outq[NV];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) g_out[k] = G[k];
-    if (!emit) return;
-    float outq[NV];
-#pragma unroll
-    for (int k = 0; k < NV; ++k) y0[k] = evolve<M>(y0[k], g_in[k], G[k], a.dt, a.dy, a.dtdy);
-    to_y<NV>(y0, outq);  // row r-2 after the y sweep
-    if constexpr (!XFirst) x_sweep_row<S, M>(outq, a);
-    if constexpr (S::kEuler) {
-        // gravity source, EE2D_KP07_dimsplit.cu:197-205
-        if (a.g > 0.0f) {
-            const float rho_v = outq[2];
-            outq[2] -= a.g * outq[0] * a.dt;
-            outq[3] -= a.g * rho_v * a.dt;
+        for (int k = 0; k < NV; ++k) y0[k] = evolve<M>(y0[k], g[k], G[k], a.dt, a.dy, a.dtdy);
+        to_y<NV>(y0, outq);  // row r-2 after the y sweep
+        if constexpr (!XFirst) x_sweep_row<S, M>(outq, a);
+        if constexpr (S::kEuler) {
+            // gravity source, EE2D_KP07_dimsplit.cu:197-205
+            if (a.g > 0.0f) {
+                const float rho_v = outq[2];
+                outq[2] -= a.g * outq[0] * a.dt;
+                outq[3] -= a.g * rho_v * a.dt;
+            }
         }
+        store_row<NV>(a, st, r, outq);
+        // the CFL minimum is taken after the y-then-x substep only: it is what
+        // computeDt reads after a full step (the last launch's values)
+        if constexpr (!XFirst) accumulate_cfl<S, M>(outq, a, st.lane_out, dt_min);
     }
-    store_row<NV>(a, st, r, outq);
-    // the CFL minimum is taken after the y-then-x substep only: it is what
-    // computeDt reads after a full step (the last launch's values)
-    if constexpr (!XFirst) accumulate_cfl<S, M>(outq, a, st.lane_out, dt_min);
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        qbt[k] = qbt_next[k];
+        g[k] = G[k];
+    }
 }
 
-// WAF: the y window holds 4 rows and there is no slope state.
+// WAF: the y window holds 4 rows (r-3 .. r) and there is no slope state; g is
+// the flux through the bottom interface of row r-2, replaced in place.
 template <class S, class M, bool XFirst>
 __device__ __forceinline__ void waf_row(const SubstepArgs& a, const Strip& st, const float (&w0)[3],
-                                        const float (&w1)[3], const float (&w2)[3], float (&w3)[3],
-                                        const float (&g_in)[3], float (&g_out)[3], int r, bool emit, float& dt_min) {
+                                        const float (&w1)[3], const float (&w2)[3], float (&w3)[3], float (&g)[3],
+                                        int r, bool emit, float& dt_min) {
     if constexpr (XFirst) x_sweep_row<S, M>(w3, a);
     float y0[3], y1[3], y2[3], y3[3], G[3];
     to_y<3>(w0, y0);
@@ -406,18 +422,19 @@ __device__ __forceinline__ void waf_row(const SubstepArgs& a, const Strip& st, c
     to_y<3>(w2, y2);
     to_y<3>(w3, y3);
     S::flux4(y0, y1, y2, y3, a, a.dy, a.dtdy, G);  // interface (r-2, r-1)
+    if (emit) {
+        float outq[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) g_out[k] = G[k];
-    if (!emit) return;
-    float outq[3];
+        for (int k = 0; k < 3; ++k) y1[k] = evolve<M>(y1[k], g[k], G[k], a.dt, a.dy, a.dtdy);
+        to_y<3>(y1, outq);
+        if constexpr (!XFirst) x_sweep_row<S, M>(outq, a);
+        store_row<3>(a, st, r, outq);
+        // the CFL minimum is taken after the y-then-x substep only: it is what
+        // computeDt reads after a full step (the last launch's values)
+        if constexpr (!XFirst) accumulate_cfl<S, M>(outq, a, st.lane_out, dt_min);
+    }
 #pragma unroll
-    for (int k = 0; k < 3; ++k) y1[k] = evolve<M>(y1[k], g_in[k], G[k], a.dt, a.dy, a.dtdy);
-    to_y<3>(y1, outq);
-    if constexpr (!XFirst) x_sweep_row<S, M>(outq, a);
-    store_row<3>(a, st, r, outq);
-    // the CFL minimum is taken after the y-then-x substep only: it is what
-    // computeDt reads after a full step (the last launch's values)
-    if constexpr (!XFirst) accumulate_cfl<S, M>(outq, a, st.lane_out, dt_min);
+    for (int k = 0; k < 3; ++k) g[k] = G[k];
 }
 
 template <int NV>
@@ -426,94 +443,88 @@ __device__ __forceinline__ void copyv(float (&d)[NV], const float (&s)[NV]) {
     for (int k = 0; k < NV; ++k) d[k] = s[k];
 }
 
+// Row march of a slope scheme over one segment.  Row m of the segment lives in
+// ring slot m % N; the steady loop is unrolled N times so that every slot index
+// is a compile-time constant (the ring stays in registers, nothing is copied).
 template <class S, class M, bool XFirst>
 __device__ __forceinline__ float march_slopes(const SubstepArgs& a, const Strip& st) {
     constexpr int NV = S::NV;
+    constexpr int PF = kRowPrefetch;
+    constexpr int N = PF + 3;  // rows r-2, r-1, r and PF rows in flight
     const float kky = M::exact ? a.dt / (2.0f * a.dy) : 0.5f * a.dtdy;
     float dt_min = M::exact ? __builtin_inff() : 0.0f;
-    // Four row slots used as a ring (the three-row window plus the row being
-    // prefetched) and two slots each for the carried face / flux; the steady
-    // loop is unrolled four times so that no slot is ever copied.
-    float A[NV], B[NV], C[NV], D[NV], P0[NV], P1[NV], G0[NV], G1[NV];
-    const int last = st.r_end - 1;
-    // prologue: rows 0, 1 (window fill), row 2 (first faces), row 3 (first flux)
-    load_row<NV>(a, st, 0, A);
-    load_row<NV>(a, st, 1, B);
-    load_row<NV>(a, st, 2, C);
-    load_row<NV>(a, st, 3, D);
+    float ring[N][NV], P[NV], G[NV];
+    const int last = st.r_end - 1;  // r_end >= 5
+    // prologue: rows 0 .. N-1; faces of row 1; then step r = 3 (flux (1,2), no output)
+#pragma unroll
+    for (int m = 0; m < N; ++m) load_row<NV>(a, st, min(m, last), ring[m]);
     if constexpr (XFirst) {
-        x_sweep_row<S, M>(A, a);
-        x_sweep_row<S, M>(B, a);
-        x_sweep_row<S, M>(C, a);
+        x_sweep_row<S, M>(ring[0], a);
+        x_sweep_row<S, M>(ring[1], a);
+        x_sweep_row<S, M>(ring[2], a);
     }
     {
         float y0[NV], y1[NV], y2[NV], qbb[NV];
-        to_y<NV>(A, y0);
-        to_y<NV>(B, y1);
-        to_y<NV>(C, y2);
-        reconstruct<S, M>(y0, y1, y2, a, kky, qbb, P0);  // faces of row 1
+        to_y<NV>(ring[0], y0);
+        to_y<NV>(ring[1], y1);
+        to_y<NV>(ring[2], y2);
+        reconstruct<S, M>(y0, y1, y2, a, kky, qbb, P);  // faces of row 1
     }
-    slope_row<S, M, XFirst>(a, st, kky, B, C, D, P0, P1, G1, G0, 3, false, dt_min);  // flux (1,2)
-    // window (C, D) = rows 2, 3; P1 = top face of row 2; G0 = flux (1,2); A <- row 4
+#pragma unroll
+    for (int k = 0; k < NV; ++k) G[k] = 0.0f;
+    load_row<NV>(a, st, min(3 + PF, last), ring[(3 + PF) % N]);
+    slope_row<S, M, XFirst>(a, st, kky, ring[1], ring[2], ring[3 % N], P, G, 3, false, dt_min);
+    // steady state: r = 4, 4 + N, ...; the step of row rr loads row rr + PF into
+    // the slot of row rr - 3, whose last use was the previous step.  Whole
+    // groups of N rows run without branches; the remainder (< N rows) keeps
+    // the same compile-time slots behind wave-uniform tests.
+    auto step = [&](int u, int rr) {
+        load_row<NV>(a, st, min(rr + PF, last), ring[(4 + u + PF) % N]);
+        slope_row<S, M, XFirst>(a, st, kky, ring[(2 + u) % N], ring[(3 + u) % N], ring[(4 + u) % N], P, G, rr, true,
+                                dt_min);
+    };
     int r = 4;
-    load_row<NV>(a, st, min(r, last), A);
-    for (; r + 4 <= st.r_end; r += 4) {
-        load_row<NV>(a, st, min(r + 1, last), B);
-        slope_row<S, M, XFirst>(a, st, kky, C, D, A, P1, P0, G0, G1, r, true, dt_min);
-        load_row<NV>(a, st, min(r + 2, last), C);
-        slope_row<S, M, XFirst>(a, st, kky, D, A, B, P0, P1, G1, G0, r + 1, true, dt_min);
-        load_row<NV>(a, st, min(r + 3, last), D);
-        slope_row<S, M, XFirst>(a, st, kky, A, B, C, P1, P0, G0, G1, r + 2, true, dt_min);
-        load_row<NV>(a, st, min(r + 4, last), A);
-        slope_row<S, M, XFirst>(a, st, kky, B, C, D, P0, P1, G1, G0, r + 3, true, dt_min);
+    for (; r + N <= st.r_end; r += N) {
+#pragma unroll
+        for (int u = 0; u < N; ++u) step(u, r + u);
     }
-    // remainder (at most 3 rows): same phase as the loop head
-    for (; r < st.r_end; ++r) {
-        load_row<NV>(a, st, min(r + 1, last), B);
-        slope_row<S, M, XFirst>(a, st, kky, C, D, A, P1, P0, G0, G1, r, true, dt_min);
-        copyv<NV>(C, D);
-        copyv<NV>(D, A);
-        copyv<NV>(A, B);
-        copyv<NV>(P1, P0);
-        copyv<NV>(G0, G1);
-    }
+#pragma unroll
+    for (int u = 0; u < N - 1; ++u)
+        if (r + u < st.r_end) step(u, r + u);
     return dt_min;
 }
 
+// Row march of WAF (4-row window): row m in ring slot m % N, as above.
 template <class S, class M, bool XFirst>
 __device__ __forceinline__ float march_waf(const SubstepArgs& a, const Strip& st) {
+    constexpr int PF = kRowPrefetch;
+    constexpr int N = PF + 4;  // rows r-3 .. r and PF rows in flight
     float dt_min = M::exact ? __builtin_inff() : 0.0f;
-    float A[3], B[3], C[3], D[3], G0[3], G1[3];
-    load_row<3>(a, st, 0, A);
-    load_row<3>(a, st, 1, B);
-    load_row<3>(a, st, 2, C);
-    load_row<3>(a, st, 3, D);
+    float ring[N][3], G[3];
+    const int last = st.r_end - 1;
+#pragma unroll
+    for (int m = 0; m < N; ++m) load_row<3>(a, st, min(m, last), ring[m]);
     if constexpr (XFirst) {
-        x_sweep_row<S, M>(A, a);
-        x_sweep_row<S, M>(B, a);
-        x_sweep_row<S, M>(C, a);
+        x_sweep_row<S, M>(ring[0], a);
+        x_sweep_row<S, M>(ring[1], a);
+        x_sweep_row<S, M>(ring[2], a);
     }
-    waf_row<S, M, XFirst>(a, st, A, B, C, D, G1, G0, 3, false, dt_min);  // flux (1,2)
-    // window (B, C, D) = rows 1..3, G0 = flux (1,2); row r goes into the freed slot
+#pragma unroll
+    for (int k = 0; k < 3; ++k) G[k] = 0.0f;
+    waf_row<S, M, XFirst>(a, st, ring[0], ring[1], ring[2], ring[3], G, 3, false, dt_min);  // flux (1,2)
+    auto step = [&](int u, int rr) {
+        load_row<3>(a, st, min(rr + PF, last), ring[(4 + u + PF) % N]);
+        waf_row<S, M, XFirst>(a, st, ring[(1 + u) % N], ring[(2 + u) % N], ring[(3 + u) % N], ring[(4 + u) % N], G,
+                              rr, true, dt_min);
+    };
     int r = 4;
-    for (; r + 4 <= st.r_end; r += 4) {
-        load_row<3>(a, st, r, A);
-        waf_row<S, M, XFirst>(a, st, B, C, D, A, G0, G1, r, true, dt_min);
-        load_row<3>(a, st, r + 1, B);
-        waf_row<S, M, XFirst>(a, st, C, D, A, B, G1, G0, r + 1, true, dt_min);
-        load_row<3>(a, st, r + 2, C);
-        waf_row<S, M, XFirst>(a, st, D, A, B, C, G0, G1, r + 2, true, dt_min);
-        load_row<3>(a, st, r + 3, D);
-        waf_row<S, M, XFirst>(a, st, A, B, C, D, G1, G0, r + 3, true, dt_min);
+    for (; r + N <= st.r_end; r += N) {
+#pragma unroll
+        for (int u = 0; u < N; ++u) step(u, r + u);
     }
-    for (; r < st.r_end; ++r) {
-        load_row<3>(a, st, r, A);
-        waf_row<S, M, XFirst>(a, st, B, C, D, A, G0, G1, r, true, dt_min);
-        copyv<3>(B, C);
-        copyv<3>(C, D);
-        copyv<3>(D, A);
-        copyv<3>(G0, G1);
-    }
+#pragma unroll
+    for (int u = 0; u < N - 1; ++u)
+        if (r + u < st.r_end) step(u, r + u);
     return dt_min;
 }
 
@@ -553,23 +564,16 @@ __global__ void __launch_bounds__(256) substep_kernel(SubstepArgs a) {
     const int gw = __builtin_amdgcn_readfirstlane(lb * a.waves_per_block + wave);
     if (lb >= a.num_blocks || gw >= a.num_waves) return;
 
-    float acc = M::exact ? __builtin_inff() : 0.0f;
-    int pos = gw * a.seg_rows;
-    const int end = min(pos + a.seg_rows, a.total_rows);
-    while (pos < end) {  // at most two segments in practice
-        const int strip_x = pos / a.rows_per_strip;
-        const int row_in = pos - strip_x * a.rows_per_strip;
-        const int count = min(end - pos, a.rows_per_strip - row_in);
-        Strip st;
-        setup_strip<S, M>(a, st, lane, strip_x, row_in, count);
-        float part;
-        if constexpr (S::kSlopes) {
-            part = (a.step == 0) ? march_slopes<S, M, true>(a, st) : march_slopes<S, M, false>(a, st);
-        } else {
-            part = (a.step == 0) ? march_waf<S, M, true>(a, st) : march_waf<S, M, false>(a, st);
-        }
-        acc = M::exact ? fminf(acc, part) : fmaxf(acc, part);
-        pos += count;
+    const int strip_x = __builtin_amdgcn_readfirstlane(gw / a.segs_per_strip);
+    const int row_in = __builtin_amdgcn_readfirstlane((gw - strip_x * a.segs_per_strip) * a.seg_rows);
+    const int count = min(a.seg_rows, a.rows_per_strip - row_in);
+    Strip st;
+    setup_strip<S, M>(a, st, lane, strip_x, row_in, count);
+    float acc;
+    if constexpr (S::kSlopes) {
+        acc = (a.step == 0) ? march_slopes<S, M, true>(a, st) : march_slopes<S, M, false>(a, st);
+    } else {
+        acc = (a.step == 0) ? march_waf<S, M, true>(a, st) : march_waf<S, M, false>(a, st);
     }
     float dt_min = acc;
     if constexpr (!M::exact) dt_min = (acc > 0.0f) ? fast::rcp(acc) : __builtin_inff();

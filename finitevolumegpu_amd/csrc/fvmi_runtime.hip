@@ -126,8 +126,9 @@ int run_substep(int scheme, SubstepArgs& a, int nvars, const fvmi_launch_t* laun
 
 __global__ void fill_kernel(float* p, size_t pitch, int w, int h, float v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
-    if (i < w && j < h) reinterpret_cast<float*>(reinterpret_cast<char*>(p) + (size_t)j * pitch)[i] = v;
+    if (i >= w) return;
+    for (int j = blockIdx.y; j < h; j += gridDim.y)  // grid.y is capped at 65535 rows
+        reinterpret_cast<float*>(reinterpret_cast<char*>(p) + (size_t)j * pitch)[i] = v;
 }
 
 __global__ void copy_kernel(float4* __restrict__ dst, const float4* __restrict__ src, size_t n4) {
@@ -416,8 +417,8 @@ int fvmi_fill_f32(float* ptr, size_t pitch_bytes, int width, int height, float v
     if (!ptr || width < 0 || height < 0 || pitch_bytes < (size_t)width * sizeof(float))
         return fail(FVMI_ERR_INVALID, "bad fill region");
     if (width == 0 || height == 0) return FVMI_OK;
-    hipLaunchKernelGGL(fill_kernel, dim3((width + 255) / 256, height), dim3(256), 0, (hipStream_t)stream, ptr,
-                       pitch_bytes, width, height, value);
+    hipLaunchKernelGGL(fill_kernel, dim3((width + 255) / 256, height < 65535 ? height : 65535), dim3(256), 0,
+                       (hipStream_t)stream, ptr, pitch_bytes, width, height, value);
     FVMI_HIP(hipGetLastError());
     return FVMI_OK;
 }
@@ -547,22 +548,22 @@ int fvmi_ensemble_count_active(const fvmi_instance_t* table, int n, int* count_d
     return FVMI_OK;
 }
 
-int fvmi_ensemble_substep(int scheme, fvmi_instance_t* table, const int* segs, int num_segs, int step,
-                          const fvmi_launch_t* launch, void* stream) {
-    if (!table || !segs || num_segs < 0) return fail(FVMI_ERR_INVALID, "bad ensemble work list");
+int fvmi_ensemble_substep(int scheme, fvmi_instance_t* table, const int* segs, const int* num_segs_dev, int waves,
+                          int step, const fvmi_launch_t* launch, void* stream) {
+    if (!table || !segs || !num_segs_dev) return fail(FVMI_ERR_INVALID, "bad ensemble work list");
     if (scheme < SCHEME_KP07 || scheme > SCHEME_EULER) return fail(FVMI_ERR_INVALID, "unknown scheme %d", scheme);
     if (step != 0 && step != 1) return fail(FVMI_ERR_INVALID, "step must be 0 or 1");
-    if (num_segs == 0) return FVMI_OK;
     EnsembleArgs e{};
     e.table = table;
     e.segs = reinterpret_cast<const int4*>(segs);
-    e.num_segs = num_segs;
+    e.num_segs = num_segs_dev;
     e.step = step;
     e.waves_per_block = (launch && launch->waves_per_block > 0) ? launch->waves_per_block : 4;
     if (e.waves_per_block > 4) return fail(FVMI_ERR_INVALID, "waves_per_block out of range [1, 4]");
     e.xcd_remap = (launch && launch->xcd_remap == 2) ? 0 : 1;
     const int math = launch ? launch->math : FVMI_MATH_FAST;
-    e.num_blocks = (num_segs + e.waves_per_block - 1) / e.waves_per_block;
+    if (waves <= 0) waves = current_cu_count() * 4 * 4;  // four waves per SIMD
+    e.num_blocks = (waves + e.waves_per_block - 1) / e.waves_per_block;
     int grid = e.num_blocks;
     if (e.xcd_remap) grid = ((grid + 7) / 8) * 8;
     hipError_t err = (math == FVMI_MATH_EXACT) ? launch_ensemble_exact(scheme, e, grid, (hipStream_t)stream)

@@ -199,10 +199,14 @@ typedef struct fvmi_instance {
 /* dt policy and CFL reset for every instance (one thread each). */
 int fvmi_ensemble_prepare(fvmi_instance_t* table, int n, void* stream);
 /* One substep of every active instance over the work list `segs` (int4 per
- * wavefront: instance, strip index, first interior row, row count; strips are
- * 60 columns wide and a segment never crosses a strip). */
-int fvmi_ensemble_substep(int scheme, fvmi_instance_t* table, const int* segs, int num_segs, int step,
-                          const fvmi_launch_t* launch, void* stream);
+ * segment: instance, strip index, first interior row, row count; strips are
+ * 60 columns wide and a segment never crosses a strip).  The number of valid
+ * segments is read on the device from *num_segs_dev, so a captured graph stays
+ * valid when the host rewrites the list and its count between replays.
+ * `waves` wavefronts are launched (<= 0: enough to fill the device); each
+ * takes segments waves apart, so any count is covered. */
+int fvmi_ensemble_substep(int scheme, fvmi_instance_t* table, const int* segs, const int* num_segs_dev, int waves,
+                          int step, const fvmi_launch_t* launch, void* stream);
 /* t += dt_step, nt += 1 for every instance that stepped. */
 int fvmi_ensemble_advance(fvmi_instance_t* table, int n, void* stream);
 /* Writes to *count_dev the number of instances that still have to step. */

@@ -94,3 +94,27 @@ def test_ensemble_matches_oracle_simulate(ctx):
     assert t == ora.t
     for k, (a, b) in enumerate(zip(ens.download(2), ora.download())):
         assert np.array_equal(a, b), (k, float(np.abs(a - b).max()))
+
+
+def test_autotuned_schedule_keeps_results(ctx):
+    """The autotuner's ensemble schedule (segment rows x waves per workgroup)
+    times every candidate, leaves its input untouched, and an ensemble run
+    with the chosen schedule still equals the plain simulate() bitwise."""
+    from finitevolumegpu_amd.Autotuner import Autotuner
+    from finitevolumegpu_amd.Ensemble import Ensemble
+    specs = _instances(4, count=6)
+    before = [f.copy() for f in specs[0]["fields"]]
+    print("case ensemble schedule", flush=True)
+    sched = Autotuner.ensemble_schedule(ctx, "kp07_dimsplit", specs, seg_rows=(None, 16), waves_per_block=(2, 4),
+                                        steps_per_graph=4, blocks=2)
+    assert len(sched["table"]) == 4 and all(row["cell_steps_per_s"] > 0 for row in sched["table"])
+    assert sched["cell_steps_per_s"] == max(row["cell_steps_per_s"] for row in sched["table"])
+    assert all(np.array_equal(a, b) for a, b in zip(before, specs[0]["fields"]))
+    ens = Ensemble(ctx, "kp07_dimsplit", specs, seg_rows=sched["seg_rows"], waves_per_block=sched["waves_per_block"])
+    ens.run()
+    for i, spec in enumerate(specs):
+        ref, _, nt_ref = _plain(ctx, spec, "fast")
+        assert ens.state(i)[1] == nt_ref
+        for k in range(3):
+            assert np.array_equal(ens.download(i)[k], ref[k]), (i, k)
+    ens.close()

@@ -107,3 +107,36 @@ def test_instance_dtype_matches_ctypes_layout():
     assert view["cfl_scale"][2] == 0.9 and view["field"][1, 1, 3] == 0x1234
     view["active"][2] = 1
     assert t[2].active == 1
+
+
+def test_device_arena_first_fit_and_coalescing(monkeypatch):
+    """DeviceArena bookkeeping with the device allocation faked: first fit,
+    256-byte rounding, oversize chunks, and merging of released neighbours."""
+    from finitevolumegpu_amd import Ensemble as E
+    base = [1 << 40]
+
+    def fake_call(name, *args):
+        assert name == "fvmi_malloc"
+        ptr, size = args
+        ptr._obj.value = base[0]
+        base[0] += size + (1 << 20)
+        return 0
+    monkeypatch.setattr(E._lib, "call", fake_call)
+    ar = E.DeviceArena(chunk_bytes=4096)
+    a = ar.alloc(100)            # -> 256 bytes at offset 0 of chunk 0
+    b = ar.alloc(1000)           # -> 1024 at 256
+    c = ar.alloc(2816)           # -> fills chunk 0 exactly (256 + 1024 + 2816 = 4096)
+    assert [h[:2] for h in (a, b, c)] == [(0, 0), (0, 256), (0, 1280)] and a[2] == 256 and b[2] == 1024
+    assert ar.free[0] == [] and ar.used == 4096
+    d = ar.alloc(10000)          # larger than a chunk: a chunk of its own size
+    assert d[:2] == (1, 0) and ar.chunks[1][1] == 10240 and ar.free[1] == []
+    ar.release(b)
+    ar.release(a)                # merges with b's block
+    assert ar.free[0] == [[0, 1280]]
+    e = ar.alloc(512)            # first fit inside the merged hole
+    assert e[:2] == (0, 0) and ar.free[0] == [[512, 768]]
+    ar.release(c)
+    ar.release(e)                # everything merges back into one block
+    assert ar.free[0] == [[0, 4096]] and ar.used == 10240
+    assert ar.address(d) == ar.chunks[1][0].value
+    assert ar.reserved() == 4096 + 10240
