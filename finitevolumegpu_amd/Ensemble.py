@@ -26,8 +26,10 @@ the plain simulator run alone with the same arithmetic policy.
 """
 
 import bisect
+import collections
 import ctypes
 import logging
+import threading
 
 import numpy as np
 
@@ -134,6 +136,65 @@ class DeviceArena:
         self.chunks, self.free, self.used = [], [], 0
 
 
+class PinnedPool:
+    """Page-locked host buffers in power-of-two size classes (at least 1 MiB),
+    reused.  Copies from page-locked memory are asynchronous, so an instance's
+    upload can queue behind a running block without holding the host thread
+    until the block ends.  Thread-safe (requests are staged on worker
+    threads)."""
+
+    MIN_BYTES = 1 << 20
+
+    def __init__(self):
+        self.lock = threading.Lock()
+        self.free = collections.defaultdict(list)
+        self.owned = []
+
+    def get(self, nbytes):
+        size = self.MIN_BYTES
+        while size < nbytes:
+            size *= 2
+        with self.lock:
+            if self.free[size]:
+                return self.free[size].pop()
+        ptr = ctypes.c_void_p()
+        _lib.call("fvmi_host_malloc", ctypes.byref(ptr), size)
+        buf = (ptr, size)
+        with self.lock:
+            self.owned.append(buf)
+        return buf
+
+    def put(self, buf):
+        with self.lock:
+            self.free[buf[1]].append(buf)
+
+    def view(self, buf, shape):
+        """float32 array of ``shape`` over the start of ``buf``."""
+        count = int(np.prod(shape))
+        arr = np.ctypeslib.as_array((ctypes.c_float * count).from_address(buf[0].value))
+        return arr.reshape(shape)
+
+    def close(self):
+        with self.lock:
+            for ptr, _ in self.owned:
+                if ptr.value:
+                    _lib.lib().fvmi_host_free(ptr)
+            self.owned, self.free = [], collections.defaultdict(list)
+
+
+def instance_initial_cfl(nvars, fields, dx, dy, g, gamma):
+    """Host CFL of an instance's initial data, float32 NumPy arithmetic as the
+    plain simulators compute it (KP07_dimsplit.py:103-105,
+    EE2D_KP07_dimsplit.py:106-108)."""
+    dx, dy = np.float32(dx), np.float32(dy)
+    if nvars == 4:
+        rho, ru, rv = fields[0], fields[1], fields[2]
+        gam = np.float32(gamma)
+        return np.float32(min(np.min(dx / (np.abs(ru / rho) + np.sqrt(gam * rho))),
+                               np.min(dy / (np.abs(rv / rho) + np.sqrt(gam * rho)))))
+    return swe_initial_cfl(fields[0], fields[1], fields[2], dx, dy, g)
+
+
 def instance_pitch(nx):
     """Row pitch in bytes of an instance's padded fields (2 ghost cells a side)."""
     return ((int(nx) + 4) * 4 + 255) // 256 * 256
@@ -143,8 +204,11 @@ class Ensemble:
     """``instances``: list of dicts with keys
     ``fields`` (list of (ny, nx) arrays), ``nx``, ``ny``, ``dx``, ``dy``, ``g``,
     ``t_end``, and optionally ``bc`` (BoundaryCondition, packed int or
-    'reflective'/'outflow'/'periodic'), ``theta``, ``gamma``, ``cfl_scale``
-    (default 0.9) and ``dt`` (fixed step, as simulate(t, dt=...))."""
+    'reflective'/'outflow'/'periodic'/'ns_reflective_ew_periodic'), ``theta``,
+    ``gamma``, ``cfl_scale`` (default 0.9), ``dt`` (fixed step, as
+    simulate(t, dt=...)) and ``cfl0`` (the initial CFL as
+    ``instance_initial_cfl`` computes it, when already known).  Fields in
+    page-locked memory upload without blocking the host."""
 
     def __init__(self, context, scheme, instances=(), math="fast", steps_per_graph=8, waves_per_block=4,
                  seg_rows=None, min_seg_rows=24, waves_per_simd=4, use_graph=True, capacity=None, stream=None,
@@ -229,18 +293,15 @@ class Ensemble:
         I.dx, I.dy, I.g = dx, dy, g
         default_theta = 1.8 if self.scheme == "hll2" else 1.3
         I.theta = np.float32(spec.get("theta", default_theta))
-        I.gamma = np.float32(spec.get("gamma", 1.4 if self.nvars == 4 else 0.0))
+        gamma = spec.get("gamma")
+        I.gamma = np.float32((1.4 if self.nvars == 4 else 0.0) if gamma is None else gamma)
         I.sqrt_g = np.sqrt(g) if g > 0 else np.float32(0.0)
         I.inv_g = np.float32(1.0) / g if g != 0 else np.float32(0.0)
         I.inv_dx = np.float32(1.0) / dx
         I.inv_dy = np.float32(1.0) / dy
-        if self.nvars == 4:
-            rho, ru, rv = fields[0], fields[1], fields[2]
-            gam = np.float32(I.gamma)
-            c0 = np.float32(min(np.min(dx / (np.abs(ru / rho) + np.sqrt(gam * rho))),
-                                np.min(dy / (np.abs(rv / rho) + np.sqrt(gam * rho)))))
-        else:
-            c0 = swe_initial_cfl(fields[0], fields[1], fields[2], dx, dy, spec["g"])
+        c0 = spec.get("cfl0")  # precomputed off the admitting thread (Serving.request_spec)
+        if c0 is None:
+            c0 = instance_initial_cfl(self.nvars, fields, dx, dy, spec["g"], I.gamma)
         I.cfl[0] = np.inf
         I.cfl[1] = c0
         I.cfl_scale = float(spec.get("cfl_scale", 0.9))
@@ -366,8 +427,11 @@ class Ensemble:
         self.planned_rows = total
 
     def _waves(self):
-        """Wavefronts of a substep launch: one per segment up to four per SIMD."""
-        return int(min(self.segs_cap, self.context.cu_count * 4 * self.waves_per_simd))
+        """Wavefronts of a substep launch: one per slot of the work list's buffer
+        (at most twice the segments when it was sized), so every segment gets
+        its own wave and the hardware hands the longest-first list to SIMDs as
+        they free up; waves past the list's length exit at once."""
+        return int(self.segs_cap)
 
     def _enqueue_steps(self, k):
         waves = self._waves()

@@ -27,7 +27,7 @@ import time
 
 import numpy as np
 
-from .Ensemble import DeviceArena, Ensemble, _bc_code
+from .Ensemble import DeviceArena, Ensemble, PinnedPool, _bc_code, instance_initial_cfl
 from .HipContext import Stream
 from .Simulator import BoundaryCondition
 
@@ -35,17 +35,37 @@ from .Simulator import BoundaryCondition
 Completed = collections.namedtuple("Completed", "id scheme arrival admitted finished steps cells fields")
 
 
-def request_spec(req):
+def request_spec(req, pinned=None):
     """The Ensemble instance dict of a request.  A request without ``fields``
-    (``initial_conditions.serving_requests``) is materialised here, at admission,
-    so the host never holds the initial data of requests still queued."""
-    if "fields" in req:
+    (``initial_conditions.serving_requests``) is materialised here, so the host
+    never holds the initial data of requests still queued.  The initial CFL is
+    computed here too, and with a ``PinnedPool`` the fields are staged in
+    page-locked memory (``_staging`` lists the buffers to return once the
+    upload has completed), so that admitting the request costs the admitting
+    thread only queued copies."""
+    if "fields" in req and pinned is None:
         return req
-    from .initial_conditions import materialise
     spec = dict(req)
-    spec.update(materialise(req))
-    spec.setdefault("nx", req["n"])
-    spec.setdefault("ny", req["n"])
+    if "fields" not in req:
+        from .initial_conditions import materialise
+        spec.update(materialise(req))
+        spec.setdefault("nx", req["n"])
+        spec.setdefault("ny", req["n"])
+    nvars = len(spec["fields"])
+    gamma = spec.get("gamma")
+    gamma = (1.4 if nvars == 4 else 0.0) if gamma is None else gamma
+    spec["cfl0"] = instance_initial_cfl(nvars, spec["fields"], spec["dx"], spec["dy"], spec["g"], np.float32(gamma))
+    if pinned is not None:
+        staged, bufs = [], []
+        for f in spec["fields"]:
+            f = np.asarray(f, dtype=np.float32)
+            buf = pinned.get(f.nbytes)
+            dst = pinned.view(buf, f.shape)
+            np.copyto(dst, f)
+            staged.append(dst)
+            bufs.append(buf)
+        spec["fields"] = staged
+        spec["_staging"] = bufs
     return spec
 
 
@@ -53,9 +73,10 @@ class _Prefetcher:
     """Materialises requests on worker threads (NumPy releases the GIL), at
     most ``depth`` ahead of the one being admitted."""
 
-    def __init__(self, workers=4, depth=8):
+    def __init__(self, workers=8, depth=32, pinned=None):
         self.pool = concurrent.futures.ThreadPoolExecutor(max_workers=workers) if workers else None
         self.depth = depth
+        self.pinned = pinned
         self.futures = {}
 
     def ahead(self, reqs):
@@ -66,7 +87,7 @@ class _Prefetcher:
             if k >= self.depth:
                 break
             if req["id"] not in self.futures:
-                self.futures[req["id"]] = self.pool.submit(request_spec, req)
+                self.futures[req["id"]] = self.pool.submit(request_spec, req, self.pinned)
 
     def ready(self, req):
         f = self.futures.get(req["id"])
@@ -74,11 +95,16 @@ class _Prefetcher:
 
     def take(self, req):
         f = self.futures.pop(req["id"], None)
-        return request_spec(req) if f is None else f.result()
+        return request_spec(req, self.pinned) if f is None else f.result()
 
     def close(self):
         if self.pool is not None:
             self.pool.shutdown(wait=True, cancel_futures=True)
+        if self.pinned is not None:  # staged requests never admitted
+            for f in self.futures.values():
+                if f.done() and f.exception() is None:
+                    for buf in f.result().get("_staging", ()):
+                        self.pinned.put(buf)
         self.futures = {}
 
 
@@ -88,7 +114,7 @@ class Server:
     ``budget_bytes`` of instance memory."""
 
     def __init__(self, context, budget_bytes=200e9, steps_per_block=8, math="fast", capacity=4096,
-                 keep_results=True, use_graph=True, prefetch_workers=4, chunk_bytes=1 << 30):
+                 keep_results=True, use_graph=True, prefetch_workers=8, chunk_bytes=1 << 30):
         self.logger = logging.getLogger(__name__)
         self.context = context
         self.budget = float(budget_bytes)
@@ -99,6 +125,8 @@ class Server:
         self.use_graph = use_graph
         self.prefetch_workers = int(prefetch_workers)
         self.arena = DeviceArena(chunk_bytes=int(min(chunk_bytes, max(self.budget, 1 << 20))))
+        self.pinned = PinnedPool()
+        self.staged = collections.defaultdict(list)  # scheme -> page-locked buffers of queued uploads
         self.pools = {}
         self.used_bytes = 0
         self.owner = {}  # (scheme, slot) -> request record
@@ -122,7 +150,9 @@ class Server:
 
     def admit(self, req, now, spec=None):
         pool = self._pool(req["scheme"])
-        slot = pool.add(spec if spec is not None else request_spec(req))
+        spec = spec if spec is not None else request_spec(req)
+        slot = pool.add(spec)
+        self.staged[req["scheme"]].extend(spec.get("_staging", ()))
         self.used_bytes += pool.bytes_of(req)
         self.owner[(req["scheme"], slot)] = {"req": req, "admitted": now}
         return slot
@@ -157,7 +187,12 @@ class Server:
             if not ok or (self.owner and not prefetch.ready(waiting[0])):
                 break
             req = waiting.popleft()
-            self.admit(req, now(), prefetch.take(req))
+            c = time.perf_counter()
+            spec = prefetch.take(req)
+            c1 = time.perf_counter()
+            self.admit(req, now(), spec)
+            self.timing["admit_data_wait_s"] += c1 - c
+            self.timing["admit_queue_s"] += time.perf_counter() - c1
             prefetch.ahead(waiting)
 
     def serve(self, requests, realtime=True, max_wall=None, progress=None):
@@ -171,7 +206,7 @@ class Server:
         total)`` is called after each retirement pass.  Returns a summary dict."""
         pending = collections.deque(sorted(requests, key=lambda r: r["arrival"]))
         waiting = collections.deque()
-        prefetch = _Prefetcher(self.prefetch_workers)
+        prefetch = _Prefetcher(self.prefetch_workers, pinned=self.pinned)
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0  # noqa: E731
         clock = time.perf_counter
@@ -195,8 +230,11 @@ class Server:
                         time.sleep(max(0.0, min(0.002, pending[0]["arrival"] - now())))
                     continue
                 c = clock()
-                for pool in busy:
-                    pool.stream.synchronize()
+                for scheme, pool in self.pools.items():
+                    if pool in busy or self.staged[scheme]:
+                        pool.stream.synchronize()
+                    for buf in self.staged.pop(scheme, ()):  # their uploads are done
+                        self.pinned.put(buf)
                 self.timing["wait_s"] += clock() - c
                 c = clock()
                 if self._retire(now) and progress is not None:
@@ -228,6 +266,7 @@ class Server:
             pool.close()
         self.pools = {}
         self.arena.close()
+        self.pinned.close()
 
 
 def make_simulator(context, req, math="fast"):
@@ -247,7 +286,7 @@ def make_simulator(context, req, math="fast"):
     return cls(context, *f, req["nx"], req["ny"], req["dx"], req["dy"], req["g"], boundary_conditions=bc, math=math)
 
 
-def run_sequential(context, requests, realtime=True, math="fast", keep_results=True, prefetch_workers=4,
+def run_sequential(context, requests, realtime=True, math="fast", keep_results=True, prefetch_workers=8,
                    progress=None):
     """Baseline: one plain simulate() per request, in arrival order (initial
     data materialised ahead on worker threads, as the server does)."""

@@ -97,7 +97,8 @@ __device__ __forceinline__ void ensemble_segment(const EnsembleArgs& e, const in
 
 // A fixed grid of waves walks the work list (wave w takes segments w, w + W,
 // ...; the list is sorted longest first), so the launch shape does not depend
-// on the list and a captured graph survives re-planning.
+// on the list and a captured graph survives re-planning.  Sized to the list's
+// buffer, the grid gives every segment its own wave.
 template <class S, class M>
 __global__ void __launch_bounds__(256) ensemble_substep_kernel(EnsembleArgs e) {
     const int lane = threadIdx.x & (kLanes - 1);

@@ -360,7 +360,7 @@ __device__ __forceinline__ void accumulate_cfl(const float (&q)[S::NV], const Su
 // rows in flight, so a load is issued kRowPrefetch rows of work before its
 // first use.
 #ifndef FVMI_ROW_PREFETCH
-#define FVMI_ROW_PREFETCH 1
+#define FVMI_ROW_PREFETCH 2
 #endif
 constexpr int kRowPrefetch = FVMI_ROW_PREFETCH;
 static_assert(kRowPrefetch >= 1 && kRowPrefetch <= 3, "row prefetch distance 1..3");

@@ -27,16 +27,26 @@ def _centres(n, d=1.0):
 def _disk(n, cx, cy, r, inside, outside=1.0):
     """n x n float32 field: ``inside`` at the cell centres with
     (x-cx)^2 < r^2 - (y-cy)^2, ``outside`` elsewhere.  Each row's disk cells are
-    one contiguous run, so the field is written row by row (cheap for the large
-    serving requests)."""
+    one contiguous run [lo, hi), found by binary search on the two monotone
+    halves of (x-cx)^2, so the field is written in whole-array passes (no
+    per-row Python loop holding the interpreter lock)."""
     x = _centres(n)
     dx2 = (x - cx) ** 2
     thr = r * r - (x - cy) ** 2
+    i0 = int(np.argmin(dx2))
+    left = dx2[:i0 + 1]   # non-increasing
+    right = dx2[i0:]      # non-decreasing
+    lo = np.searchsorted(-left, -thr, side="right")       # count of left cells with dx2 >= thr
+    hi = i0 + np.searchsorted(right, thr, side="left")    # i0 + count of right cells with dx2 < thr
     h = np.full((n, n), outside, dtype=np.float32)
-    for j in np.flatnonzero(thr > 0.0):
-        idx = np.flatnonzero(dx2 < thr[j])
-        if idx.size:
-            h[j, idx[0]:idx[-1] + 1] = inside
+    rows = np.flatnonzero(hi > lo)
+    if rows.size:
+        j0, j1 = rows[0], rows[-1] + 1  # the disk's bounding box
+        c0, c1 = int(lo[j0:j1].min()), int(hi[j0:j1].max())
+        cols = np.arange(c0, c1)
+        mask = (cols[None, :] >= lo[j0:j1, None]) & (cols[None, :] < hi[j0:j1, None])
+        box = h[j0:j1, c0:c1]
+        box[mask] = np.float32(inside)
     return h
 
 

@@ -140,3 +140,19 @@ def test_device_arena_first_fit_and_coalescing(monkeypatch):
     assert ar.free[0] == [[0, 4096]] and ar.used == 10240
     assert ar.address(d) == ar.chunks[1][0].value
     assert ar.reserved() == 4096 + 10240
+
+
+def test_disk_generator_matches_direct_mask():
+    """The row-run disk of the ensemble/serving workloads equals the direct
+    cell-centre test (x-cx)^2 < r^2 - (y-cy)^2, including disks clipped by the
+    domain and empty ones."""
+    from finitevolumegpu_amd.initial_conditions import _centres, _disk
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        n = int(rng.integers(2, 160))
+        cx, cy = rng.uniform(-5, n + 5, size=2)
+        r = rng.uniform(0, n)
+        x = _centres(n)
+        want = np.where((x[None, :] - cx) ** 2 < r * r - (x[:, None] - cy) ** 2, 2.5, 1.0).astype(np.float32)
+        got = _disk(n, cx, cy, r, 2.5)
+        assert got.dtype == np.float32 and np.array_equal(got, want), (n, cx, cy, r)
