@@ -245,6 +245,79 @@ class DimsplitSimulator(BaseSimulator):
             self.cfl_data = None
 
 
+class UnsplitSimulator(BaseSimulator):
+    """Shared body of the LxF, HLL, FORCE and KP07 wrappers: fields with
+    ``gc`` ghost cells, one launch per substep with the full dt (LxF.py:106-
+    121, KP07.py:108-126), swap after each.  The two-float CFL buffer
+    alternates slots between launches (see fvmi.h); computeDt is
+    ``dt_factor`` times the last launch's minimum."""
+
+    scheme = None  # "lxf" | "hll" | "force" | "kp07"
+    gc = 1
+    nvars = 3
+    dt_factor = 0.5
+    default_waves_per_block = 4
+
+    def _init_unsplit(self, fields, cfl_host, tile_rows, waves_per_block, math, xcd_remap):
+        if self.context.autotuner is not None and tile_rows is None and waves_per_block is None:
+            peak = self.context.autotuner.get_peak_performance(self.__class__)
+            tile_rows = int(peak["tile_rows"])
+            waves_per_block = int(peak["waves_per_block"])
+        if math not in _MATH:
+            raise ValueError("math must be 'fast' or 'exact'")
+        self.math = math
+        self.launch = _lib.LaunchConfig(int(tile_rows or 0),
+                                        int(waves_per_block or self.default_waves_per_block),
+                                        _MATH[math], 1 if xcd_remap else 2)
+        self.gc_x = self.gc
+        self.gc_y = self.gc
+        self.u0 = Common.ArakawaA2D(self.stream, self.nx, self.ny, self.gc, self.gc, fields)
+        self.u1 = Common.ArakawaA2D(self.stream, self.nx, self.ny, self.gc, self.gc, [None] * len(fields))
+        ptr = ctypes.c_void_p()
+        _lib.call("fvmi_malloc", ctypes.byref(ptr), 8)
+        self.cfl_data = ptr
+        self.cfl_last = 1  # slot holding the latest minimum: the initial value
+        init = np.array([np.inf, cfl_host], dtype=np.float32)
+        _lib.call("fvmi_memcpy_async", self.cfl_data, init.ctypes.data, 8, _lib.MEMCPY_HTOD, self.stream.handle)
+        self.stream.synchronize()
+
+    def cfl_pointer(self):
+        return self.cfl_data.value
+
+    def substep(self, dt, step_number):
+        slot = 1 - self.cfl_last
+        self._launch(np.float32(dt), int(step_number), self.u0, self.u1, self.cfl_pointer(), slot)
+        self.cfl_last = slot
+        self.u0, self.u1 = self.u1, self.u0
+
+    def _field_args(self, state):
+        args = []
+        for k in range(self.nvars):
+            args += [state[k].data_ptr, state[k].pitch]
+        return args
+
+    def getOutput(self):
+        return self.u0
+
+    def check(self):
+        self.u0.check(self.stream)
+        self.u1.check(self.stream)
+
+    def computeDt(self):
+        """``dt_factor`` x the CFL minimum of the last launch (LxF.py:123-125,
+        HLL.py:122-124, FORCE.py:127-129, KP07.py:136-138)."""
+        host = np.zeros(2, dtype=np.float32)
+        _lib.call("fvmi_memcpy_async", host.ctypes.data, self.cfl_data, 8, _lib.MEMCPY_DTOH, self.stream.handle)
+        self.stream.synchronize()
+        return float(host[self.cfl_last]) * self.dt_factor
+
+    def __del__(self):
+        ptr = getattr(self, "cfl_data", None)
+        if ptr is not None and ptr.value and _lib._lib is not None:
+            _lib.lib().fvmi_free(ptr)
+            self.cfl_data = None
+
+
 def swe_initial_cfl(h0, hu0, hv0, dx, dy, g):
     """Host CFL of the initial data (KP07_dimsplit.py:103-105), float32."""
     h0 = np.asarray(h0)

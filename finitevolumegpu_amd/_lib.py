@@ -105,6 +105,14 @@ SIGNATURES = {
     "fvmi_euler_kp07_dimsplit_substep": (c_int, [c_int, c_int, c_float, c_float, c_float, c_float, c_float, c_float,
                                                  c_int, c_int] + [c_void_p, c_int] * 8
                                          + [c_void_p, c_int, c_int, c_int, c_int, P(LaunchConfig), c_void_p]),
+    "fvmi_swe_lxf_substep": (c_int, [c_int, c_int, c_float, c_float, c_float, c_float, c_int]
+                             + [c_void_p, c_int] * 6 + [c_void_p, c_int, P(LaunchConfig), c_void_p]),
+    "fvmi_swe_hll_substep": (c_int, [c_int, c_int, c_float, c_float, c_float, c_float, c_int]
+                             + [c_void_p, c_int] * 6 + [c_void_p, c_int, P(LaunchConfig), c_void_p]),
+    "fvmi_swe_force_substep": (c_int, [c_int, c_int, c_float, c_float, c_float, c_float, c_int]
+                               + [c_void_p, c_int] * 6 + [c_void_p, c_int, P(LaunchConfig), c_void_p]),
+    "fvmi_swe_kp07_substep": (c_int, [c_int, c_int, c_float, c_float, c_float, c_float, c_float, c_int, c_int]
+                              + [c_void_p, c_int] * 6 + [c_void_p, c_int, P(LaunchConfig), c_void_p]),
     "fvmi_halo_copy_periodic_x": (c_int, [c_int, c_int, c_int, P(c_void_p), P(c_int), c_void_p]),
     "fvmi_halo_exchange_periodic": (c_int, [c_int, c_int, c_int, P(c_void_p), P(c_int), c_int, c_int, c_void_p]),
     "fvmi_field_stats": (c_int, [c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),

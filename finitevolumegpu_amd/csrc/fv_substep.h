@@ -74,21 +74,28 @@ struct AxisMap {
 // Memory index read for padded index k along an axis of n interior cells
 // (common.h:247-288 and the mirror rules of :133-239).  Indices past the padded
 // array are clamped as readBlock does (those lanes/rows are never stored).
-__device__ __forceinline__ AxisMap map_axis(int k, int n, int bc_lo, int bc_hi) {
-    if (k < 2) {
+// GC ghost cells a side: ghost k < GC mirrors cell 2*GC-1-k (bc*Reflective,
+// common.h:133-239, for any gc), periodic wraps by n.
+template <int GC>
+__device__ __forceinline__ AxisMap map_axis_gc(int k, int n, int bc_lo, int bc_hi) {
+    if (k < GC) {
         if (bc_lo == BC_PERIODIC) return {k + n, false};
-        if (bc_lo == BC_REFLECTIVE) return {3 - k, true};
-        if (bc_lo == BC_OUTFLOW) return {2, false};
+        if (bc_lo == BC_REFLECTIVE) return {2 * GC - 1 - k, true};
+        if (bc_lo == BC_OUTFLOW) return {GC, false};
         return {k, false};
     }
-    if (k >= n + 2) {
-        if (k > n + 3) k = n + 3;
+    if (k >= n + GC) {
+        if (k > n + 2 * GC - 1) k = n + 2 * GC - 1;
         if (bc_hi == BC_PERIODIC) return {k - n, false};
-        if (bc_hi == BC_REFLECTIVE) return {2 * n + 3 - k, true};
-        if (bc_hi == BC_OUTFLOW) return {n + 1, false};
+        if (bc_hi == BC_REFLECTIVE) return {2 * (n + GC) - 1 - k, true};
+        if (bc_hi == BC_OUTFLOW) return {n + GC - 1, false};
         return {k, false};
     }
     return {k, false};
+}
+
+__device__ __forceinline__ AxisMap map_axis(int k, int n, int bc_lo, int bc_hi) {
+    return map_axis_gc<2>(k, n, bc_lo, bc_hi);
 }
 
 // Cross-lane moves: the value held by lane-1 / lane+1, as one DPP row move

@@ -9,6 +9,7 @@
 
 #include "../../include/fvmi.h"
 #include "fv_launch.h"
+#include "fv_unsplit.h"
 
 using namespace fvmi;
 
@@ -252,6 +253,69 @@ __global__ void field_stats_kernel(int nx, int ny, const float* f, int pitch, do
     }
 }
 
+// Shared checks and launch of the unsplit / first-order schemes (gc ghost cells).
+int run_unsplit(int scheme, UnsplitArgs& a, int gc, const fvmi_launch_t* launch, void* stream) {
+    if (a.nx < 2 || a.ny < 2) return fail(FVMI_ERR_INVALID, "grid %dx%d too small (need nx, ny >= 2)", a.nx, a.ny);
+    if ((size_t)(a.ny + 2 * gc) * (size_t)(a.nx + 2 * gc) > (size_t)1 << 30)
+        return fail(FVMI_ERR_INVALID, "grid %dx%d too large", a.nx, a.ny);
+    if (int rc = check_bc(a.bc)) return rc;
+    if (a.cfl_slot != 0 && a.cfl_slot != 1) return fail(FVMI_ERR_INVALID, "cfl_slot must be 0 or 1");
+    const bool centred = scheme == USCHEME_LXF || scheme == USCHEME_FORCE;
+    if (!std::isfinite(a.dt) || (centred ? !(a.dt > 0.0f) : !(a.dt >= 0.0f)))
+        return fail(FVMI_ERR_INVALID, centred ? "dt must be finite and > 0" : "dt must be finite and >= 0");
+    if (!(a.dx > 0.0f) || !(a.dy > 0.0f)) return fail(FVMI_ERR_INVALID, "dx, dy must be > 0");
+    const int min_pitch = (a.nx + 2 * gc) * (int)sizeof(float);
+    for (int k = 0; k < 3; ++k) {
+        if (!a.in[k] || !a.out[k]) return fail(FVMI_ERR_INVALID, "field pointer %d is NULL", k);
+        if (a.in[k] == a.out[k]) return fail(FVMI_ERR_INVALID, "field %d: input and output alias", k);
+        if (a.in_pitch[k] < min_pitch || a.out_pitch[k] < min_pitch || (a.in_pitch[k] & 3) || (a.out_pitch[k] & 3))
+            return fail(FVMI_ERR_INVALID, "field %d: pitch must be >= %d bytes and a multiple of 4", k, min_pitch);
+        if ((size_t)(a.ny + 2 * gc) * (size_t)a.in_pitch[k] >= ((size_t)1 << 32) ||
+            (size_t)(a.ny + 2 * gc) * (size_t)a.out_pitch[k] >= ((size_t)1 << 32))
+            return fail(FVMI_ERR_INVALID, "field %d: (ny+%d)*pitch must be below 4 GiB", k, 2 * gc);
+    }
+    SubstepArgs shape{};
+    int math = 0;
+    if (int rc = fill_launch(launch, shape, &math)) return rc;
+    a.seg_rows = shape.seg_rows;
+    a.waves_per_block = shape.waves_per_block;
+    a.xcd_remap = shape.xcd_remap;
+    a.hg = 0.5f * a.g;
+    a.sqrt_g = std::sqrt(a.g > 0.0f ? a.g : 0.0f);
+    a.inv_dx = 1.0f / a.dx;
+    a.inv_dy = 1.0f / a.dy;
+    a.dtdx = a.dt / a.dx;
+    a.dtdy = a.dt / a.dy;
+    if (centred) {
+        a.lxf_kx = a.dx / (4.0f * a.dt);
+        a.lxf_ky = a.dy / (4.0f * a.dt);
+        a.lf_kx = a.dx / (2.0f * a.dt);
+        a.lf_ky = a.dy / (2.0f * a.dt);
+        a.lw_kx = a.dt / (2.0f * a.dx);
+        a.lw_ky = a.dt / (2.0f * a.dy);
+    }
+    const int grid = plan_unsplit(a, gc, current_cu_count());
+    hipError_t e = (math == FVMI_MATH_EXACT) ? launch_unsplit_exact(scheme, a, grid, (hipStream_t)stream)
+                                             : launch_unsplit_fast(scheme, a, grid, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "unsplit launch");
+    return FVMI_OK;
+}
+
+UnsplitArgs first_order_args(int nx, int ny, float dx, float dy, float dt, float g, int bc, const float* h0,
+                             int h0_pitch, const float* hu0, int hu0_pitch, const float* hv0, int hv0_pitch, float* h1,
+                             int h1_pitch, float* hu1, int hu1_pitch, float* hv1, int hv1_pitch, float* cfl,
+                             int cfl_slot) {
+    UnsplitArgs a{};
+    a.nx = nx, a.ny = ny, a.dx = dx, a.dy = dy, a.dt = dt, a.g = g, a.bc = bc;
+    a.in[0] = h0, a.in[1] = hu0, a.in[2] = hv0;
+    a.in_pitch[0] = h0_pitch, a.in_pitch[1] = hu0_pitch, a.in_pitch[2] = hv0_pitch;
+    a.out[0] = h1, a.out[1] = hu1, a.out[2] = hv1;
+    a.out_pitch[0] = h1_pitch, a.out_pitch[1] = hu1_pitch, a.out_pitch[2] = hv1_pitch;
+    a.cfl = cfl;
+    a.cfl_slot = cfl_slot;
+    return a;
+}
+
 }  // namespace
 
 extern "C" {
@@ -450,6 +514,48 @@ int fvmi_swe_kp07_dimsplit_substep(int nx, int ny, float dx, float dy, float dt,
     a.out_pitch[0] = h1_pitch, a.out_pitch[1] = hu1_pitch, a.out_pitch[2] = hv1_pitch;
     a.cfl = cfl;
     return run_substep(SCHEME_KP07, a, 3, launch, stream);
+}
+
+int fvmi_swe_lxf_substep(int nx, int ny, float dx, float dy, float dt, float g, int boundary_conditions,
+                         const float* h0, int h0_pitch, const float* hu0, int hu0_pitch, const float* hv0,
+                         int hv0_pitch, float* h1, int h1_pitch, float* hu1, int hu1_pitch, float* hv1, int hv1_pitch,
+                         float* cfl, int cfl_slot, const fvmi_launch_t* launch, void* stream) {
+    UnsplitArgs a = first_order_args(nx, ny, dx, dy, dt, g, boundary_conditions, h0, h0_pitch, hu0, hu0_pitch, hv0,
+                                     hv0_pitch, h1, h1_pitch, hu1, hu1_pitch, hv1, hv1_pitch, cfl, cfl_slot);
+    return run_unsplit(USCHEME_LXF, a, 1, launch, stream);
+}
+
+int fvmi_swe_hll_substep(int nx, int ny, float dx, float dy, float dt, float g, int boundary_conditions,
+                         const float* h0, int h0_pitch, const float* hu0, int hu0_pitch, const float* hv0,
+                         int hv0_pitch, float* h1, int h1_pitch, float* hu1, int hu1_pitch, float* hv1, int hv1_pitch,
+                         float* cfl, int cfl_slot, const fvmi_launch_t* launch, void* stream) {
+    UnsplitArgs a = first_order_args(nx, ny, dx, dy, dt, g, boundary_conditions, h0, h0_pitch, hu0, hu0_pitch, hv0,
+                                     hv0_pitch, h1, h1_pitch, hu1, hu1_pitch, hv1, hv1_pitch, cfl, cfl_slot);
+    return run_unsplit(USCHEME_HLL, a, 1, launch, stream);
+}
+
+int fvmi_swe_force_substep(int nx, int ny, float dx, float dy, float dt, float g, int boundary_conditions,
+                           const float* h0, int h0_pitch, const float* hu0, int hu0_pitch, const float* hv0,
+                           int hv0_pitch, float* h1, int h1_pitch, float* hu1, int hu1_pitch, float* hv1,
+                           int hv1_pitch, float* cfl, int cfl_slot, const fvmi_launch_t* launch, void* stream) {
+    UnsplitArgs a = first_order_args(nx, ny, dx, dy, dt, g, boundary_conditions, h0, h0_pitch, hu0, hu0_pitch, hv0,
+                                     hv0_pitch, h1, h1_pitch, hu1, hu1_pitch, hv1, hv1_pitch, cfl, cfl_slot);
+    return run_unsplit(USCHEME_FORCE, a, 1, launch, stream);
+}
+
+int fvmi_swe_kp07_substep(int nx, int ny, float dx, float dy, float dt, float g, float theta, int step_order,
+                          int boundary_conditions, const float* h0, int h0_pitch, const float* hu0, int hu0_pitch,
+                          const float* hv0, int hv0_pitch, float* h1, int h1_pitch, float* hu1, int hu1_pitch,
+                          float* hv1, int hv1_pitch, float* cfl, int cfl_slot, const fvmi_launch_t* launch,
+                          void* stream) {
+    const int step = step_order >> 16, order = step_order & 0xFFFF;
+    if (order < 1 || order > 2 || step < 0 || step >= order)
+        return fail(FVMI_ERR_INVALID, "step_order: step %d of order %d (order 1 or 2)", step, order);
+    UnsplitArgs a = first_order_args(nx, ny, dx, dy, dt, g, boundary_conditions, h0, h0_pitch, hu0, hu0_pitch, hv0,
+                                     hv0_pitch, h1, h1_pitch, hu1, hu1_pitch, hv1, hv1_pitch, cfl, cfl_slot);
+    a.theta = theta;
+    a.stage2 = (order == 2 && step == 1) ? 1 : 0;
+    return run_unsplit(USCHEME_KP07, a, 2, launch, stream);
 }
 
 int fvmi_swe_hll2_substep(int nx, int ny, float dx, float dy, float dt, float g, float theta, int step,

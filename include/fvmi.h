@@ -136,6 +136,39 @@ int fvmi_swe_waf_substep(int nx, int ny, float dx, float dy, float dt, float g, 
                          int hv0_pitch, float* h1, int h1_pitch, float* hu1, int hu1_pitch, float* hv1,
                          int hv1_pitch, float* cfl, const fvmi_launch_t* launch, void* stream);
 
+/* ---- the reference's other shallow-water schemes --------------------------
+ * One launch of LxFKernel (SWE2D_LxF.cu:93-110), HLLKernel (SWE2D_HLL.cu:
+ * 100-115) or FORCEKernel (SWE2D_FORCE.cu:88-105): one full first-order step
+ * on fields with ONE ghost cell a side ((ny+2) rows of pitch >= (nx+2)*4
+ * bytes).  Arguments in the reference's order; then `cfl_slot` (0 or 1): the
+ * launch minimises cfl[cfl_slot] with the per-cell admissible dt of the cells
+ * it writes and sets cfl[1 - cfl_slot] to +inf, so a caller alternating the
+ * slot reads the last launch's minimum without a reset in between.  LxF and
+ * FORCE need dt > 0 (their fluxes divide by it). */
+int fvmi_swe_lxf_substep(int nx, int ny, float dx, float dy, float dt, float g, int boundary_conditions,
+                         const float* h0, int h0_pitch, const float* hu0, int hu0_pitch, const float* hv0,
+                         int hv0_pitch, float* h1, int h1_pitch, float* hu1, int hu1_pitch, float* hv1, int hv1_pitch,
+                         float* cfl, int cfl_slot, const fvmi_launch_t* launch, void* stream);
+int fvmi_swe_hll_substep(int nx, int ny, float dx, float dy, float dt, float g, int boundary_conditions,
+                         const float* h0, int h0_pitch, const float* hu0, int hu0_pitch, const float* hv0,
+                         int hv0_pitch, float* h1, int h1_pitch, float* hu1, int hu1_pitch, float* hv1, int hv1_pitch,
+                         float* cfl, int cfl_slot, const fvmi_launch_t* launch, void* stream);
+int fvmi_swe_force_substep(int nx, int ny, float dx, float dy, float dt, float g, int boundary_conditions,
+                           const float* h0, int h0_pitch, const float* hu0, int hu0_pitch, const float* hv0,
+                           int hv0_pitch, float* h1, int h1_pitch, float* hu1, int hu1_pitch, float* hv1,
+                           int hv1_pitch, float* cfl, int cfl_slot, const fvmi_launch_t* launch, void* stream);
+
+/* KP07Kernel, SWE2D_KP07.cu:120-146: one stage of the unsplit KP07 scheme
+ * with TVD RK2, fields with TWO ghost cells a side.  step_order = step << 16 |
+ * order (Simulator.stepOrderToCodedInt); the second stage of order 2 writes
+ * 0.5 * (h1_old + Q), reading the output buffer first.  cfl_slot as above;
+ * the CFL is that of Q before the average, as the reference computes it. */
+int fvmi_swe_kp07_substep(int nx, int ny, float dx, float dy, float dt, float g, float theta, int step_order,
+                          int boundary_conditions, const float* h0, int h0_pitch, const float* hu0, int hu0_pitch,
+                          const float* hv0, int hv0_pitch, float* h1, int h1_pitch, float* hu1, int hu1_pitch,
+                          float* hv1, int hv1_pitch, float* cfl, int cfl_slot, const fvmi_launch_t* launch,
+                          void* stream);
+
 /* Euler KP07DimsplitKernel, EE2D_KP07_dimsplit.cu:126-154.  x0,y0,x1,y1 select
  * an interior sub-area [x0,x1) x [y0,y1) to update (x1 == 0 -> nx, y1 == 0 -> ny,
  * as the reference's defaults); boundary conditions are applied at the true
