@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark: KP07_dimsplit shallow water, 4096x4096 float32, on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--schemes kp07,hll2,waf,ee2d]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--schemes kp07,hll2,waf,ee2d,lxf,hll,force,kp07u]
                     [--no-cpu-baseline] [--ensemble 512] [--serve 2000 --serve-rate 200]
 
 A "step" is one full time step of the plain Simulator API (two fused substep
@@ -18,8 +18,10 @@ stream.  One JSON line is printed:
                     rocprofv3 counter summary in profiles/ when present
 * cpu_baseline      the float32 NumPy oracle (one core) on a bounded sample of
                     the same workload (512x512 crop of the generator, a few steps)
-* schemes           the same measurement for HLL2 and WAF at 4096^2 and
-                    EE2D_KP07_dimsplit at 8192^2 (64 B per cell-step)
+* schemes           the same measurement for HLL2 and WAF at 4096^2,
+                    EE2D_KP07_dimsplit at 8192^2 (64 B per cell-step) and the
+                    reference's other schemes LxF, HLL, FORCE (24 B per
+                    cell-step) and the unsplit KP07 with RK2 (60 B) at 4096^2
 * ensemble          512 dam breaks of uneven size in the batched engine against
                     the plain simulate() loop over the same instances
 * serving           the continuous-batching server on 2000 Poisson-arriving
@@ -38,8 +40,6 @@ sys.path.insert(0, ROOT)
 
 import numpy as np  # noqa: E402
 
-SWE_BYTES_PER_CELL_SUBSTEP = 24  # 3 float32 fields, read once + written once
-EULER_BYTES_PER_CELL_SUBSTEP = 32
 HBM_PEAK_GBS = 8000.0
 
 
@@ -49,7 +49,7 @@ def _parse():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=4096, help="grid side of the SWE workload")
-    ap.add_argument("--schemes", default="kp07,hll2,waf,ee2d")
+    ap.add_argument("--schemes", default="kp07,hll2,waf,ee2d,lxf,hll,force,kp07u")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--waves-per-block", type=int, default=0)
@@ -116,21 +116,42 @@ def _swe_sim(ctx, scheme, n, args):
         from finitevolumegpu_amd.KP07_dimsplit import KP07_dimsplit as C
     elif scheme == "hll2":
         from finitevolumegpu_amd.HLL2 import HLL2 as C
-    else:
+    elif scheme == "waf":
         from finitevolumegpu_amd.WAF import WAF as C
+    elif scheme == "lxf":
+        from finitevolumegpu_amd.LxF import LxF as C
+    elif scheme == "hll":
+        from finitevolumegpu_amd.HLL import HLL as C
+    elif scheme == "force":
+        from finitevolumegpu_amd.FORCE import FORCE as C
+    elif scheme == "kp07u":
+        from finitevolumegpu_amd.KP07 import KP07 as C
+    else:
+        raise ValueError("unknown scheme %r" % scheme)
     sim = C(ctx, h, hu, hv, n, n, 1.0, 1.0, 9.81, **kw)
     dt = np.float32(sim.computeDt() * 0.9)
     return sim, dt
 
 
+# HBM bytes per cell of one full step as the algorithm needs them (fields read
+# once and written once per launch; the second RK2 stage of the unsplit KP07
+# also reads the output buffer), and launches per step
+STEP_BYTES = {"kp07": (48, 2), "hll2": (48, 2), "waf": (48, 2), "lxf": (24, 1), "hll": (24, 1), "force": (24, 1),
+              "kp07u": (60, 2), "ee2d": (64, 2)}
+
+
 def _euler_sim(ctx, n):
+    """One rank's 8192^2 subdomain: Rayleigh-Taylor, reflective north/south,
+    periodic east/west through the one-rank MPISimulator (Dirichlet edges and
+    an on-device halo exchange after every substep)."""
     from finitevolumegpu_amd.EE2D_KP07_dimsplit import EE2D_KP07_dimsplit
+    from finitevolumegpu_amd.MPISimulator import MPISimulator
     from finitevolumegpu_amd.Simulator import BoundaryCondition
     from finitevolumegpu_amd.initial_conditions import rayleigh_taylor
     rho, ru, rv, E, dx, dy = rayleigh_taylor(n, n, seed=0)
     T = BoundaryCondition.Type
     bc = BoundaryCondition({"north": T.Reflective, "south": T.Reflective, "east": T.Periodic, "west": T.Periodic})
-    sim = EE2D_KP07_dimsplit(ctx, rho, ru, rv, E, n, n, dx, dy, 0.1, 1.4, boundary_conditions=bc)
+    sim = MPISimulator(EE2D_KP07_dimsplit(ctx, rho, ru, rv, E, n, n, dx, dy, 0.1, 1.4, boundary_conditions=bc))
     c = np.sqrt(1.4 * 2.6 / 1.0)
     dt = np.float32(0.45 * dx / c)
     return sim, dt
@@ -140,23 +161,23 @@ def _measure(ctx, kind, args):
     if kind == "ee2d":
         n = 8192
         sim, dt = _euler_sim(ctx, n)
-        bpc = EULER_BYTES_PER_CELL_SUBSTEP
         steps = max(1, args.steps // 3)
     else:
         n = args.n
         sim, dt = _swe_sim(ctx, kind, n, args)
-        bpc = SWE_BYTES_PER_CELL_SUBSTEP
         steps = args.steps
+    step_bytes, per_step = STEP_BYTES[kind]
     ms, wall = _time_steps(sim, dt, steps, args.warmup, ctx)
     out = sim.download()
     finite = all(bool(np.isfinite(a).all()) for a in out)
-    launches = 2 * steps
+    launches = per_step * steps
     avg_launch_ms = ms / launches
     cells = n * n
     res = {
         "n": n, "steps": steps, "ms_per_step": ms / steps, "cell_updates_per_s": cells * steps / (ms * 1e-3),
-        "avg_launch_ms": avg_launch_ms, "algorithmic_gbs": bpc * cells / (avg_launch_ms * 1e-3) / 1e9,
-        "wall_s": wall, "finite": finite, "launch": [sim.launch.tile_rows, sim.launch.waves_per_block],
+        "avg_launch_ms": avg_launch_ms,
+        "algorithmic_gbs": step_bytes * cells * steps / (ms * 1e-3) / 1e9,
+        "wall_s": wall, "finite": finite, "launch": [getattr(sim, "sim", sim).launch.tile_rows, getattr(sim, "sim", sim).launch.waves_per_block],
     }
     del sim
     return res

@@ -666,7 +666,11 @@ int fvmi_ensemble_substep(int scheme, fvmi_instance_t* table, const int* segs, c
     e.step = step;
     e.waves_per_block = (launch && launch->waves_per_block > 0) ? launch->waves_per_block : 4;
     if (e.waves_per_block > 4) return fail(FVMI_ERR_INVALID, "waves_per_block out of range [1, 4]");
-    e.xcd_remap = (launch && launch->xcd_remap == 2) ? 0 : 1;
+    // The work list is sorted longest first: dispatch order (block i on XCD
+    // i % 8) spreads it evenly over the XCDs, whereas the contiguous remap of
+    // the single-grid kernel would give XCD 0 the longest segments and leave
+    // the last XCDs with the empty tail of the list.
+    e.xcd_remap = 0;
     const int math = launch ? launch->math : FVMI_MATH_FAST;
     if (waves <= 0) waves = current_cu_count() * 4 * 4;  // four waves per SIMD
     e.num_blocks = (waves + e.waves_per_block - 1) / e.waves_per_block;

@@ -237,7 +237,9 @@ int fvmi_ensemble_prepare(fvmi_instance_t* table, int n, void* stream);
  * segments is read on the device from *num_segs_dev, so a captured graph stays
  * valid when the host rewrites the list and its count between replays.
  * `waves` wavefronts are launched (<= 0: enough to fill the device); each
- * takes segments waves apart, so any count is covered. */
+ * takes segments waves apart, so any count is covered.  launch->xcd_remap is
+ * not used: blocks keep dispatch order, which spreads the longest-first list
+ * evenly over the XCDs. */
 int fvmi_ensemble_substep(int scheme, fvmi_instance_t* table, const int* segs, const int* num_segs_dev, int waves,
                           int step, const fvmi_launch_t* launch, void* stream);
 /* t += dt_step, nt += 1 for every instance that stepped. */
