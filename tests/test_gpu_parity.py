@@ -102,3 +102,16 @@ def test_launch_shapes_agree(ctx, tile_rows, waves):
             sim.step(np.float32(0.05))
         for k, (a, b) in enumerate(zip(sim.download(), ref.download())):
             assert np.array_equal(a, b), (math, k, float(np.abs(a - b).max()))
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+@pytest.mark.parametrize("shape", [(2, 2), (3, 5), (60, 2), (61, 3), (64, 7), (121, 2)])
+def test_exact_bitwise_edge_shapes(ctx, scheme, shape):
+    """Smallest grids and widths at the 60-column strip boundaries, one and
+    several segments, every boundary mix of the suite."""
+    nx, ny = shape
+    print("case edge", scheme, shape, flush=True)
+    for bcname in ("reflective", "mixed"):
+        sim, ora = _run_pair(ctx, scheme, nx, ny, BCS[bcname], steps=3, math="exact", tile_rows=1)
+        for k, (a, b) in enumerate(zip(sim.download(), ora.download())):
+            assert np.array_equal(a, b), (scheme, shape, bcname, k, float(np.abs(a - b).max()))

@@ -155,3 +155,15 @@ def test_simulate_dt_policy(ctx, scheme):
         ora.step(dt)
     for k, (a, b) in enumerate(zip(sim.download(), ora.download())):
         assert np.array_equal(a, b), (scheme, k)
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+@pytest.mark.parametrize("shape", [(2, 2), (3, 5), (60, 2), (62, 3), (63, 4), (125, 2)])
+def test_exact_bitwise_edge_shapes(ctx, scheme, shape):
+    """Smallest grids and widths at the 62- and 60-column strip boundaries."""
+    nx, ny = shape
+    print("case edge", scheme, shape, flush=True)
+    for bcname in ("reflective", "mixed"):
+        sim, ora = _pair(ctx, scheme, nx, ny, BCS[bcname], 3, "exact", tile_rows=1)
+        for k, (a, b) in enumerate(zip(sim.download(), ora.download())):
+            assert np.array_equal(a, b), (scheme, shape, bcname, k, float(np.abs(a - b).max()))
