@@ -100,7 +100,7 @@ __device__ __forceinline__ void ensemble_segment(const EnsembleArgs& e, const in
 // on the list and a captured graph survives re-planning.  Sized to the list's
 // buffer, the grid gives every segment its own wave.
 template <class S, class M>
-__global__ void __launch_bounds__(256) ensemble_substep_kernel(EnsembleArgs e) {
+__global__ void __launch_bounds__(256, 4) ensemble_substep_kernel(EnsembleArgs e) {
     const int lane = threadIdx.x & (kLanes - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     int lb = blockIdx.x;

@@ -72,67 +72,6 @@ __device__ __forceinline__ void swe_hll_flux(const float ql[3], const float qr[3
     }
 }
 
-__device__ __forceinline__ float desing(float x, float eps, float inv_2eps) {
-    return __builtin_copysignf(1.0f, x) *
-           __builtin_fmaxf(__builtin_fabsf(x), __builtin_fminf(x * x * inv_2eps + 0.5f * eps, eps));
-}
-
-__device__ __forceinline__ float gen_minmod(float r, float theta) {
-    return __builtin_fmaxf(0.0f, __builtin_fminf(theta * r, __builtin_fminf((1.0f + r) * 0.5f, theta)));
-}
-
-// WAF flux (SWECommon.h:131-210); dtdx = dt/dx, inv_g = 1/g
-__device__ __forceinline__ void swe_waf_flux(const float q_l2[3], const float q_l1[3], const float q_r1[3],
-                                             const float q_r2[3], float hg, float sg, float inv_g, float dtdx,
-                                             float out[3]) {
-    const float h_l = q_l1[0], h_r = q_r1[0], h_l2 = q_l2[0], h_r2 = q_r2[0];
-    const float sl = rsq(h_l), sr = rsq(h_r), sl2 = rsq(h_l2), sr2 = rsq(h_r2);
-    const float il = sl * sl, ir = sr * sr, il2 = sl2 * sl2, ir2 = sr2 * sr2;
-    const float u_l = q_l1[1] * il, u_r = q_r1[1] * ir, u_l2 = q_l2[1] * il2, u_r2 = q_r2[1] * ir2;
-    const float v_l = q_l1[2] * il, v_r = q_r1[2] * ir, v_l2 = q_l2[2] * il2, v_r2 = q_r2[2] * ir2;
-    const float c_l = sg * h_l * sl, c_r = sg * h_r * sr, c_l2 = sg * h_l2 * sl2, c_r2 = sg * h_r2 * sr2;
-    auto hstar = [&](float ul, float ur, float cl, float cr) {
-        const float t = 0.5f * (cl + cr) + 0.25f * (ul - ur);
-        return t * t * inv_g;
-    };
-    const float h_dag_l = hstar(u_l2, u_l, c_l2, c_l);
-    const float h_dag = hstar(u_l, u_r, c_l, c_r);
-    const float h_dag_r = hstar(u_r, u_r2, c_r, c_r2);
-    const float q_l = (h_dag > h_l) ? sqrt(0.5f * ((h_dag + h_l) * h_dag)) * il : 1.0f;
-    const float q_r = (h_dag > h_r) ? sqrt(0.5f * ((h_dag + h_r) * h_dag)) * ir : 1.0f;
-    const float S_l = u_l - c_l * q_l;
-    const float S_r = u_r + c_r * q_r;
-    const float dl = h_l * (u_l - S_l), dr = h_r * (u_r - S_r);
-    const float S_star = (S_l * dr - S_r * dl) * rcp(dr - dl);
-    // shared reciprocal of (S_l - S_star)(S_r - S_star)
-    const float el = S_l - S_star, er = S_r - S_star;
-    const float ie = rcp(el * er);
-    const float s_l = h_l * (S_l - u_l) * (er * ie);
-    const float s_r = h_r * (S_r - u_r) * (el * ie);
-    const float F_1[3] = {q_l1[1], q_l1[1] * u_l + hg * h_l * h_l, q_l1[2] * u_l};
-    const float F_4[3] = {q_r1[1], q_r1[1] * u_r + hg * h_r * h_r, q_r1[2] * u_r};
-    const float Qsl[3] = {s_l, s_l * S_star, s_l * v_l};
-    const float Qsr[3] = {s_r, s_r * S_star, s_r * v_r};
-    const float c_1 = S_l * dtdx, c_2 = S_star * dtdx, c_3 = S_r * dtdx;
-    const float eps = 1.0e-6f, inv_2eps = 0.5e6f;
-    const float n1 = desing((c_1 > 0.0f) ? (h_dag_l - h_l2) : (h_dag_r - h_r), eps, inv_2eps);
-    const float n2 = desing((c_2 > 0.0f) ? (v_l - v_l2) : (v_r2 - v_r), eps, inv_2eps);
-    const float n3 = desing((c_3 > 0.0f) ? (h_l - h_dag_l) : (h_r2 - h_dag_r), eps, inv_2eps);
-    const float d1 = desing(h_dag - h_l, eps, inv_2eps);
-    const float d2 = desing(v_r - v_l, eps, inv_2eps);
-    const float d3 = desing(h_r - h_dag, eps, inv_2eps);
-    const float r_1 = n1 * rcp(d1), r_2 = n2 * rcp(d2), r_3 = n3 * rcp(d3);
-    const float A_1 = __builtin_copysignf(1.0f, c_1) * (1.0f - (1.0f - __builtin_fabsf(c_1)) * gen_minmod(r_1, 1.9f));
-    const float A_2 = __builtin_copysignf(1.0f, c_2) * (1.0f - (1.0f - __builtin_fabsf(c_2)) * gen_minmod(r_2, 1.9f));
-    const float A_3 = __builtin_copysignf(1.0f, c_3) * (1.0f - (1.0f - __builtin_fabsf(c_3)) * gen_minmod(r_3, 1.9f));
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const float F_2 = F_1[k] + S_l * (Qsl[k] - q_l1[k]);
-        const float F_3 = F_4[k] + S_r * (Qsr[k] - q_r1[k]);
-        out[k] = 0.5f * (F_1[k] + F_4[k]) - 0.5f * (A_1 * (F_2 - F_1[k]) + A_2 * (F_3 - F_2) + A_3 * (F_4[k] - F_3));
-    }
-}
-
 // 1 / (admissible dt) of one cell, SWE (SWECommon.h:497-504): max over the two
 // directions of (|u| + sqrt(g h)) / d.
 __device__ __forceinline__ float swe_cell_rate(const float q[3], float sg, float idx, float idy) {

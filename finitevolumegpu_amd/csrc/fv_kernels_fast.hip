@@ -1,11 +1,22 @@
 // Fast arithmetic instantiation (hardware reciprocal / square root, contraction
-// allowed).  The production path.
+// allowed).  The production path; KP07_dimsplit and HLL2 live in
+// fv_kernels_fast_ilp.hip.
 #include "fv_launch.h"
 #include "fv_unsplit.h"
 
 namespace fvmi {
 hipError_t launch_substep_fast(int scheme, const SubstepArgs& a, int grid, hipStream_t stream) {
-    return launch_substep<MathFast>(scheme, a, grid, stream);
+    switch (scheme) {
+        case SCHEME_KP07:
+        case SCHEME_HLL2:
+            return launch_substep_fast_ilp(scheme, a, grid, stream);
+        case SCHEME_WAF:
+            return launch_substep_scheme<SweWAF<MathFast>, MathFast>(a, grid, stream);
+        case SCHEME_EULER:
+            return launch_substep_scheme<EulerKP07<MathFast>, MathFast>(a, grid, stream);
+        default:
+            return hipErrorInvalidValue;
+    }
 }
 hipError_t launch_ensemble_fast(int scheme, const EnsembleArgs& e, int grid, hipStream_t stream) {
     return launch_ensemble_substep<MathFast>(scheme, e, grid, stream);

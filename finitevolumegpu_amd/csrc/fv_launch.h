@@ -36,6 +36,13 @@ inline int plan_substep(SubstepArgs& a, int cu_count, int waves_per_simd = 4, in
     return grid;
 }
 
+// One scheme's kernel (each translation unit instantiates the ones it owns).
+template <class S, class M>
+hipError_t launch_substep_scheme(const SubstepArgs& a, int grid, hipStream_t stream) {
+    hipLaunchKernelGGL((substep_kernel<S, M>), dim3(grid), dim3(kLanes * a.waves_per_block), 0, stream, a);
+    return hipGetLastError();
+}
+
 template <class M>
 hipError_t launch_substep(int scheme, SubstepArgs a, int grid, hipStream_t stream) {
     const dim3 block(kLanes * a.waves_per_block);
@@ -85,5 +92,8 @@ hipError_t launch_ensemble_exact(int scheme, const EnsembleArgs& e, int grid, hi
 hipError_t launch_ensemble_fast(int scheme, const EnsembleArgs& e, int grid, hipStream_t stream);
 hipError_t launch_substep_exact(int scheme, const SubstepArgs& a, int grid, hipStream_t stream);
 hipError_t launch_substep_fast(int scheme, const SubstepArgs& a, int grid, hipStream_t stream);
+// KP07_dimsplit and HLL2 in fast arithmetic (fv_kernels_fast_ilp.hip, built
+// with the ILP-maximising scheduler)
+hipError_t launch_substep_fast_ilp(int scheme, const SubstepArgs& a, int grid, hipStream_t stream);
 
 }  // namespace fvmi

@@ -180,85 +180,152 @@ __device__ __forceinline__ float compute_h_star(float h_l, float h_r, float u_l,
     return M::div(h_tmp * h_tmp, g);
 }
 
-// WAF_1D_flux, SWECommon.h:131-210: interface between q_l1 and q_r1
+// ---------------------------------------------------------------------------
+// WAF in pieces: per-cell quantities, h* per interface, and the flux from the
+// two cells at the interface plus what it needs of the outer two.  A march
+// computes each cell's quantities and each interface's h* once and shares
+// them between the (up to four) fluxes that use them; the values are those
+// swe_waf_flux computes for itself, so the exact policy stays bitwise equal.
+// ---------------------------------------------------------------------------
+
+struct WafCell {
+    float h, hu, hv;  // state in sweep order (depth, normal, tangential momentum)
+    float u, v, c;    // normal and tangential velocity, sqrt(g h)
+    float ih;         // 1/h (fast policy only)
+};
+
 template <class M>
-__device__ __forceinline__ void swe_waf_flux(const float q_l2[3], const float q_l1[3], const float q_r1[3],
-                                             const float q_r2[3], float g, float dx, float dt, float out[3]) {
-    const float h_l = q_l1[0], h_r = q_r1[0];
-    const float h_l2 = q_l2[0], h_r2 = q_r2[0];
-    float u_l, u_r, u_l2, u_r2, v_l, v_r, v_l2, v_r2;
+__device__ __forceinline__ WafCell waf_cell(const float q[3], float g, float sqrt_g) {
+    WafCell p;
+    p.h = q[0];
+    p.hu = q[1];
+    p.hv = q[2];
     if constexpr (M::exact) {
-        u_l = q_l1[1] / h_l;
-        u_r = q_r1[1] / h_r;
-        u_l2 = q_l2[1] / h_l2;
-        u_r2 = q_r2[1] / h_r2;
-        v_l = q_l1[2] / h_l;
-        v_r = q_r1[2] / h_r;
-        v_l2 = q_l2[2] / h_l2;
-        v_r2 = q_r2[2] / h_r2;
+        p.u = q[1] / q[0];
+        p.v = q[2] / q[0];
+        p.c = M::sqrt(g * q[0]);
+        p.ih = 0.0f;
     } else {
-        const float il = M::rcp(h_l), ir = M::rcp(h_r), il2 = M::rcp(h_l2), ir2 = M::rcp(h_r2);
-        u_l = q_l1[1] * il;
-        u_r = q_r1[1] * ir;
-        u_l2 = q_l2[1] * il2;
-        u_r2 = q_r2[1] * ir2;
-        v_l = q_l1[2] * il;
-        v_r = q_r1[2] * ir;
-        v_l2 = q_l2[2] * il2;
-        v_r2 = q_r2[2] * ir2;
+        const float s = __builtin_amdgcn_rsqf(q[0]);
+        p.ih = s * s;
+        p.u = q[1] * p.ih;
+        p.v = q[2] * p.ih;
+        p.c = sqrt_g * q[0] * s;
     }
-    const float c_l = M::sqrt(g * h_l);
-    const float c_r = M::sqrt(g * h_r);
-    const float c_l2 = M::sqrt(g * h_l2);
-    const float c_r2 = M::sqrt(g * h_r2);
-    const float h_dag_l = compute_h_star<M>(h_l2, h_l, u_l2, u_l, c_l2, c_l, g);
-    const float h_dag = compute_h_star<M>(h_l, h_r, u_l, u_r, c_l, c_r, g);
-    const float h_dag_r = compute_h_star<M>(h_r, h_r2, u_r, u_r2, c_r, c_r2, g);
-    const float q_l_tmp = M::div(M::sqrt(0.5f * ((h_dag + h_l) * h_dag)), h_l);
-    const float q_r_tmp = M::div(M::sqrt(0.5f * ((h_dag + h_r) * h_dag)), h_r);
-    const float q_l = (h_dag > h_l) ? q_l_tmp : 1.0f;
-    const float q_r = (h_dag > h_r) ? q_r_tmp : 1.0f;
-    const float S_l = u_l - c_l * q_l;
-    const float S_r = u_r + c_r * q_r;
-    const float S_star = M::div(S_l * h_r * (u_r - S_r) - S_r * h_l * (u_l - S_l),
-                                h_r * (u_r - S_r) - h_l * (u_l - S_l));
-    const float s_l = M::div(h_l * (S_l - u_l), S_l - S_star);
-    const float s_r = M::div(h_r * (S_r - u_r), S_r - S_star);
-    const float Q_star_l[3] = {s_l * 1.0f, s_l * S_star, s_l * v_l};
-    const float Q_star_r[3] = {s_r * 1.0f, s_r * S_star, s_r * v_r};
-    float F_1[3], F_4[3], F_2[3], F_3[3];
-    swe_F<M>(q_l1, g, F_1);
-    swe_F<M>(q_r1, g, F_4);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        F_2[k] = F_1[k] + S_l * (Q_star_l[k] - q_l1[k]);
-        F_3[k] = F_4[k] + S_r * (Q_star_r[k] - q_r1[k]);
-    }
-    float c_1, c_2, c_3;
+    return p;
+}
+
+// the cell seen by the other sweep direction (normal and tangential swapped)
+__device__ __forceinline__ WafCell waf_transpose(const WafCell& p) {
+    WafCell t = p;
+    t.hu = p.hv;
+    t.hv = p.hu;
+    t.u = p.v;
+    t.v = p.u;
+    return t;
+}
+
+// computeHStar (SWECommon.h:114-121) of the interface between l and r
+template <class M>
+__device__ __forceinline__ float waf_hstar(const WafCell& l, const WafCell& r, float g, float inv_g) {
     if constexpr (M::exact) {
-        c_1 = S_l * dt / dx;
-        c_2 = S_star * dt / dx;
-        c_3 = S_r * dt / dx;
+        return compute_h_star<M>(l.h, r.h, l.u, r.u, l.c, r.c, g);
     } else {
-        const float k = dt * M::rcp(dx);
-        c_1 = S_l * k;
-        c_2 = S_star * k;
-        c_3 = S_r * k;
+        const float t = 0.5f * (l.c + r.c) + 0.25f * (l.u - r.u);
+        return t * t * inv_g;
     }
-    const float eps = 1.0e-6f;
-    const float r_1 = M::div(desingularize<M>((c_1 > 0.0f) ? (h_dag_l - h_l2) : (h_dag_r - h_r), eps),
-                             desingularize<M>(h_dag - h_l, eps));
-    const float r_2 = M::div(desingularize<M>((c_2 > 0.0f) ? (v_l - v_l2) : (v_r2 - v_r), eps),
-                             desingularize<M>(v_r - v_l, eps));
-    const float r_3 = M::div(desingularize<M>((c_3 > 0.0f) ? (h_l - h_dag_l) : (h_r2 - h_dag_r), eps),
-                             desingularize<M>(h_r - h_dag, eps));
-    const float A_1 = __builtin_copysignf(1.0f, c_1) * limiter_to_waf(generalized_minmod(r_1, 1.9f), c_1);
-    const float A_2 = __builtin_copysignf(1.0f, c_2) * limiter_to_waf(generalized_minmod(r_2, 1.9f), c_2);
-    const float A_3 = __builtin_copysignf(1.0f, c_3) * limiter_to_waf(generalized_minmod(r_3, 1.9f), c_3);
+}
+
+// WAF_1D_flux (SWECommon.h:131-210) between cells l (= Q_l1) and r (= Q_r1),
+// given depth and tangential velocity of the outer cells (Q_l2, Q_r2) and h*
+// of the interfaces (l2, l), (l, r), (r, r2).  dtd = dt / d (fast policy).
+template <class M>
+__device__ __forceinline__ void waf_flux_core(const WafCell& L, const WafCell& R, float h_l2, float v_l2, float h_r2,
+                                              float v_r2, float h_dag_l, float h_dag, float h_dag_r, float g,
+                                              float hg, float d, float dt, float dtd, float out[3]) {
+    const float h_l = L.h, h_r = R.h, u_l = L.u, u_r = R.u, v_l = L.v, v_r = R.v, c_l = L.c, c_r = R.c;
+    const float q_l1[3] = {L.h, L.hu, L.hv};
+    const float q_r1[3] = {R.h, R.hu, R.hv};
+    if constexpr (M::exact) {
+        const float q_l_tmp = M::div(M::sqrt(0.5f * ((h_dag + h_l) * h_dag)), h_l);
+        const float q_r_tmp = M::div(M::sqrt(0.5f * ((h_dag + h_r) * h_dag)), h_r);
+        const float q_l = (h_dag > h_l) ? q_l_tmp : 1.0f;
+        const float q_r = (h_dag > h_r) ? q_r_tmp : 1.0f;
+        const float S_l = u_l - c_l * q_l;
+        const float S_r = u_r + c_r * q_r;
+        const float S_star = M::div(S_l * h_r * (u_r - S_r) - S_r * h_l * (u_l - S_l),
+                                    h_r * (u_r - S_r) - h_l * (u_l - S_l));
+        const float s_l = M::div(h_l * (S_l - u_l), S_l - S_star);
+        const float s_r = M::div(h_r * (S_r - u_r), S_r - S_star);
+        const float Q_star_l[3] = {s_l * 1.0f, s_l * S_star, s_l * v_l};
+        const float Q_star_r[3] = {s_r * 1.0f, s_r * S_star, s_r * v_r};
+        float F_1[3], F_4[3], F_2[3], F_3[3];
+        swe_F<M>(q_l1, g, F_1);
+        swe_F<M>(q_r1, g, F_4);
 #pragma unroll
-    for (int k = 0; k < 3; ++k)
-        out[k] = 0.5f * (F_1[k] + F_4[k]) -
-                 0.5f * (A_1 * (F_2[k] - F_1[k]) + A_2 * (F_3[k] - F_2[k]) + A_3 * (F_4[k] - F_3[k]));
+        for (int k = 0; k < 3; ++k) {
+            F_2[k] = F_1[k] + S_l * (Q_star_l[k] - q_l1[k]);
+            F_3[k] = F_4[k] + S_r * (Q_star_r[k] - q_r1[k]);
+        }
+        const float c_1 = S_l * dt / d;
+        const float c_2 = S_star * dt / d;
+        const float c_3 = S_r * dt / d;
+        const float eps = 1.0e-6f;
+        const float r_1 = M::div(desingularize<M>((c_1 > 0.0f) ? (h_dag_l - h_l2) : (h_dag_r - h_r), eps),
+                                 desingularize<M>(h_dag - h_l, eps));
+        const float r_2 = M::div(desingularize<M>((c_2 > 0.0f) ? (v_l - v_l2) : (v_r2 - v_r), eps),
+                                 desingularize<M>(v_r - v_l, eps));
+        const float r_3 = M::div(desingularize<M>((c_3 > 0.0f) ? (h_l - h_dag_l) : (h_r2 - h_dag_r), eps),
+                                 desingularize<M>(h_r - h_dag, eps));
+        const float A_1 = __builtin_copysignf(1.0f, c_1) * limiter_to_waf(generalized_minmod(r_1, 1.9f), c_1);
+        const float A_2 = __builtin_copysignf(1.0f, c_2) * limiter_to_waf(generalized_minmod(r_2, 1.9f), c_2);
+        const float A_3 = __builtin_copysignf(1.0f, c_3) * limiter_to_waf(generalized_minmod(r_3, 1.9f), c_3);
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            out[k] = 0.5f * (F_1[k] + F_4[k]) -
+                     0.5f * (A_1 * (F_2[k] - F_1[k]) + A_2 * (F_3[k] - F_2[k]) + A_3 * (F_4[k] - F_3[k]));
+    } else {
+        const float il = L.ih, ir = R.ih;
+        const float q_l = (h_dag > h_l) ? M::sqrt(0.5f * ((h_dag + h_l) * h_dag)) * il : 1.0f;
+        const float q_r = (h_dag > h_r) ? M::sqrt(0.5f * ((h_dag + h_r) * h_dag)) * ir : 1.0f;
+        const float S_l = u_l - c_l * q_l;
+        const float S_r = u_r + c_r * q_r;
+        const float dl = h_l * (u_l - S_l), dr = h_r * (u_r - S_r);
+        const float S_star = (S_l * dr - S_r * dl) * M::rcp(dr - dl);
+        // shared reciprocal of (S_l - S_star)(S_r - S_star)
+        const float el = S_l - S_star, er = S_r - S_star;
+        const float ie = M::rcp(el * er);
+        const float s_l = h_l * (S_l - u_l) * (er * ie);
+        const float s_r = h_r * (S_r - u_r) * (el * ie);
+        const float F_1[3] = {L.hu, L.hu * u_l + hg * h_l * h_l, L.hv * u_l};
+        const float F_4[3] = {R.hu, R.hu * u_r + hg * h_r * h_r, R.hv * u_r};
+        const float Qsl[3] = {s_l, s_l * S_star, s_l * v_l};
+        const float Qsr[3] = {s_r, s_r * S_star, s_r * v_r};
+        const float c_1 = S_l * dtd, c_2 = S_star * dtd, c_3 = S_r * dtd;
+        const float eps = 1.0e-6f, inv_2eps = 0.5e6f;
+        auto desing = [&](float x) {
+            return __builtin_copysignf(1.0f, x) *
+                   __builtin_fmaxf(__builtin_fabsf(x), __builtin_fminf(x * x * inv_2eps + 0.5f * eps, eps));
+        };
+        auto gmm = [](float r) {
+            return __builtin_fmaxf(0.0f, __builtin_fminf(1.9f * r, __builtin_fminf((1.0f + r) * 0.5f, 1.9f)));
+        };
+        const float n1 = desing((c_1 > 0.0f) ? (h_dag_l - h_l2) : (h_dag_r - h_r));
+        const float n2 = desing((c_2 > 0.0f) ? (v_l - v_l2) : (v_r2 - v_r));
+        const float n3 = desing((c_3 > 0.0f) ? (h_l - h_dag_l) : (h_r2 - h_dag_r));
+        const float r_1 = n1 * M::rcp(desing(h_dag - h_l));
+        const float r_2 = n2 * M::rcp(desing(v_r - v_l));
+        const float r_3 = n3 * M::rcp(desing(h_r - h_dag));
+        const float A_1 = __builtin_copysignf(1.0f, c_1) * (1.0f - (1.0f - __builtin_fabsf(c_1)) * gmm(r_1));
+        const float A_2 = __builtin_copysignf(1.0f, c_2) * (1.0f - (1.0f - __builtin_fabsf(c_2)) * gmm(r_2));
+        const float A_3 = __builtin_copysignf(1.0f, c_3) * (1.0f - (1.0f - __builtin_fabsf(c_3)) * gmm(r_3));
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float F_2 = F_1[k] + S_l * (Qsl[k] - q_l1[k]);
+            const float F_3 = F_4[k] + S_r * (Qsr[k] - q_r1[k]);
+            out[k] = 0.5f * (F_1[k] + F_4[k]) - 0.5f * (A_1 * (F_2 - F_1[k]) + A_2 * (F_3 - F_2) + A_3 * (F_4[k] - F_3));
+        }
+    }
 }
 
 // writeCfl per-cell value, SWECommon.h:497-504

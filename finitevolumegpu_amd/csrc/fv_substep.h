@@ -108,6 +108,19 @@ __device__ __forceinline__ float from_right(float v) {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
 }
 
+// the WafCell of lane+1
+__device__ __forceinline__ WafCell waf_shift_right(const WafCell& p) {
+    WafCell r;
+    r.h = from_right(p.h);
+    r.hu = from_right(p.hu);
+    r.hv = from_right(p.hv);
+    r.u = from_right(p.u);
+    r.v = from_right(p.v);
+    r.c = from_right(p.c);
+    r.ih = from_right(p.ih);
+    return r;
+}
+
 __device__ __forceinline__ float wave_min(float v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v = fminf(v, __shfl_xor(v, off, kLanes));
@@ -170,15 +183,9 @@ struct SweHLL2 : SweBase<M> {
 
 template <class M>
 struct SweWAF : SweBase<M> {
+    // the flux is assembled by the march from shared pieces (waf_cell,
+    // waf_hstar, waf_flux_core in fv_math.h)
     static constexpr bool kSlopes = false;
-    // interface flux from the four cells around it; d = dx or dy, dtd = dt/d
-    __device__ static void flux4(const float l2[3], const float l1[3], const float r1[3], const float r2[3],
-                                 const SubstepArgs& a, float d, float dtd, float o[3]) {
-        if constexpr (M::exact)
-            swe_waf_flux<M>(l2, l1, r1, r2, a.g, d, a.dt, o);
-        else
-            fast::swe_waf_flux(l2, l1, r1, r2, a.hg, a.sqrt_g, a.inv_g, dtd, o);
-    }
 };
 
 template <class M>
@@ -280,14 +287,13 @@ __device__ __forceinline__ void x_sweep_row(float q[S::NV], const SubstepArgs& a
         for (int k = 0; k < NV; ++k) qbl_r[k] = from_right(qbl[k]);
         S::flux_x(qbr, qbl_r, a, F);  // interface (c, c+1)
     } else {
-        float l[3], r[3], rr[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            l[k] = from_left(q[k]);
-            r[k] = from_right(q[k]);
-            rr[k] = from_right(r[k]);
-        }
-        S::flux4(l, q, r, rr, a, a.dx, a.dtdx, F);
+        // WAF: each lane's cell quantities and the h* of its right interface
+        // once; neighbours' through DPP (Q_l2 = c-1, Q_r1 = c+1, Q_r2 = c+2)
+        const WafCell P = waf_cell<M>(q, a.g, a.sqrt_g);
+        const WafCell R = waf_shift_right(P);
+        const float hd = waf_hstar<M>(P, R, a.g, a.inv_g);
+        waf_flux_core<M>(P, R, from_left(P.h), from_left(P.v), from_right(R.h), from_right(R.v), from_left(hd), hd,
+                         from_right(hd), a.g, a.hg, a.dx, a.dt, a.dtdx, F);
     }
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
@@ -416,21 +422,29 @@ __device__ __forceinline__ void slope_row(const SubstepArgs& a, const Strip& st,
     }
 }
 
-// WAF: the y window holds 4 rows (r-3 .. r) and there is no slope state; g is
-// the flux through the bottom interface of row r-2, replaced in place.
+// WAF: the y window holds 4 rows (r-3 .. r) and there is no slope state.
+// Each ring slot carries its row's WafCell (sweep-y view), computed once when
+// the row is ready; hd[0], hd[1] carry h* of the interfaces (r-3, r-2) and
+// (r-2, r-1); g is the flux through the bottom interface of row r-2.  All
+// carried values are replaced in place.
 template <class S, class M, bool XFirst>
 __device__ __forceinline__ void waf_row(const SubstepArgs& a, const Strip& st, const float (&w0)[3],
-                                        const float (&w1)[3], const float (&w2)[3], float (&w3)[3], float (&g)[3],
-                                        int r, bool emit, float& dt_min) {
+                                        const float (&w1)[3], float (&w3)[3], const WafCell& c0, const WafCell& c1,
+                                        const WafCell& c2, WafCell& c3, float (&hd)[2], float (&g)[3], int r,
+                                        bool emit, float& dt_min) {
+    (void)w0;
     if constexpr (XFirst) x_sweep_row<S, M>(w3, a);
-    float y0[3], y1[3], y2[3], y3[3], G[3];
-    to_y<3>(w0, y0);
-    to_y<3>(w1, y1);
-    to_y<3>(w2, y2);
-    to_y<3>(w3, y3);
-    S::flux4(y0, y1, y2, y3, a, a.dy, a.dtdy, G);  // interface (r-2, r-1)
+    {
+        float y3[3];
+        to_y<3>(w3, y3);
+        c3 = waf_cell<M>(y3, a.g, a.sqrt_g);
+    }
+    const float hd_r = waf_hstar<M>(c2, c3, a.g, a.inv_g);  // interface (r-1, r)
+    float G[3];
+    waf_flux_core<M>(c1, c2, c0.h, c0.v, c3.h, c3.v, hd[0], hd[1], hd_r, a.g, a.hg, a.dy, a.dt, a.dtdy, G);
     if (emit) {
-        float outq[3];
+        float y1[3], outq[3];
+        to_y<3>(w1, y1);
 #pragma unroll
         for (int k = 0; k < 3; ++k) y1[k] = evolve<M>(y1[k], g[k], G[k], a.dt, a.dy, a.dtdy);
         to_y<3>(y1, outq);
@@ -440,6 +454,8 @@ __device__ __forceinline__ void waf_row(const SubstepArgs& a, const Strip& st, c
         // computeDt reads after a full step (the last launch's values)
         if constexpr (!XFirst) accumulate_cfl<S, M>(outq, a, st.lane_out, dt_min);
     }
+    hd[0] = hd[1];
+    hd[1] = hd_r;
 #pragma unroll
     for (int k = 0; k < 3; ++k) g[k] = G[k];
 }
@@ -501,28 +517,34 @@ __device__ __forceinline__ float march_slopes(const SubstepArgs& a, const Strip&
     return dt_min;
 }
 
-// Row march of WAF (4-row window): row m in ring slot m % N, as above.
+// Row march of WAF (4-row window): row m and its WafCell in ring slot m % N.
 template <class S, class M, bool XFirst>
 __device__ __forceinline__ float march_waf(const SubstepArgs& a, const Strip& st) {
     constexpr int PF = kRowPrefetch;
     constexpr int N = PF + 4;  // rows r-3 .. r and PF rows in flight
     float dt_min = M::exact ? __builtin_inff() : 0.0f;
-    float ring[N][3], G[3];
+    float ring[N][3], G[3], hd[2];
+    WafCell cell[N];
     const int last = st.r_end - 1;
 #pragma unroll
     for (int m = 0; m < N; ++m) load_row<3>(a, st, min(m, last), ring[m]);
-    if constexpr (XFirst) {
-        x_sweep_row<S, M>(ring[0], a);
-        x_sweep_row<S, M>(ring[1], a);
-        x_sweep_row<S, M>(ring[2], a);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        if constexpr (XFirst) x_sweep_row<S, M>(ring[m], a);
+        float y[3];
+        to_y<3>(ring[m], y);
+        cell[m] = waf_cell<M>(y, a.g, a.sqrt_g);
     }
+    hd[0] = waf_hstar<M>(cell[0], cell[1], a.g, a.inv_g);  // interface (0, 1)
+    hd[1] = waf_hstar<M>(cell[1], cell[2], a.g, a.inv_g);  // interface (1, 2)
 #pragma unroll
     for (int k = 0; k < 3; ++k) G[k] = 0.0f;
-    waf_row<S, M, XFirst>(a, st, ring[0], ring[1], ring[2], ring[3], G, 3, false, dt_min);  // flux (1,2)
+    waf_row<S, M, XFirst>(a, st, ring[0], ring[1], ring[3], cell[0], cell[1], cell[2], cell[3], hd, G, 3, false,
+                          dt_min);  // flux (1,2)
     auto step = [&](int u, int rr) {
         load_row<3>(a, st, min(rr + PF, last), ring[(4 + u + PF) % N]);
-        waf_row<S, M, XFirst>(a, st, ring[(1 + u) % N], ring[(2 + u) % N], ring[(3 + u) % N], ring[(4 + u) % N], G,
-                              rr, true, dt_min);
+        waf_row<S, M, XFirst>(a, st, ring[(1 + u) % N], ring[(2 + u) % N], ring[(4 + u) % N], cell[(1 + u) % N],
+                              cell[(2 + u) % N], cell[(3 + u) % N], cell[(4 + u) % N], hd, G, rr, true, dt_min);
     };
     int r = 4;
     for (; r + N <= st.r_end; r += N) {
@@ -556,7 +578,7 @@ __device__ __forceinline__ void setup_strip(const SubstepArgs& a, Strip& st, int
 }
 
 template <class S, class M>
-__global__ void __launch_bounds__(256) substep_kernel(SubstepArgs a) {
+__global__ void __launch_bounds__(256, 4) substep_kernel(SubstepArgs a) {
     const int lane = threadIdx.x & (kLanes - 1);
     // wave index made scalar so that every row address is computed in SGPRs
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);

@@ -429,7 +429,7 @@ __device__ __forceinline__ float march_kp07u(const UnsplitArgs& a, const UStrip<
 }
 
 template <class S, class M>
-__global__ void __launch_bounds__(256) unsplit_kernel(UnsplitArgs a) {
+__global__ void __launch_bounds__(256, 4) unsplit_kernel(UnsplitArgs a) {
     constexpr int GC = S::GC;
     const int lane = threadIdx.x & (kLanes - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
