@@ -577,8 +577,13 @@ __device__ __forceinline__ void setup_strip(const SubstepArgs& a, Strip& st, int
     }
 }
 
+// at least this many waves per SIMD must fit (the plan asks for four)
+#ifndef FVMI_SUBSTEP_MIN_WAVES
+#define FVMI_SUBSTEP_MIN_WAVES 4
+#endif
+
 template <class S, class M>
-__global__ void __launch_bounds__(256, 4) substep_kernel(SubstepArgs a) {
+__global__ void __launch_bounds__(256, FVMI_SUBSTEP_MIN_WAVES) substep_kernel(SubstepArgs a) {
     const int lane = threadIdx.x & (kLanes - 1);
     // wave index made scalar so that every row address is computed in SGPRs
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
