@@ -99,3 +99,42 @@ def test_missing_library_raises(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "absent.so"))
     with pytest.raises(_lib.FvmiError):
         _lib.lib()
+
+
+def _first_order(fn, nx=16, ny=16, dt=0.1, slot=0, pitch=256, bc=0):
+    p = 0x1000
+    return getattr(_lib.lib(), fn)(nx, ny, 1.0, 1.0, dt, 9.81, bc, p, pitch, p + 0x100000, pitch, p + 0x200000, pitch,
+                                   p + 0x300000, pitch, p + 0x400000, pitch, p + 0x500000, pitch, None, slot, None,
+                                   None)
+
+
+@pytest.mark.parametrize("fn", ["fvmi_swe_lxf_substep", "fvmi_swe_hll_substep", "fvmi_swe_force_substep"])
+def test_first_order_validation(fn):
+    assert _first_order(fn, nx=1) == -1 and "too small" in _err()
+    assert _first_order(fn, slot=2) == -1 and "cfl_slot" in _err()
+    assert _first_order(fn, pitch=64) == -1 and "pitch" in _err()  # (16 + 2) * 4 = 72 bytes needed
+    assert _first_order(fn, nx=40000, ny=40000) == -1 and "too large" in _err()
+    neumann = (1 << 24) | (3 << 16) | (3 << 8) | 3
+    assert _first_order(fn, bc=neumann) == -3
+    if fn != "fvmi_swe_hll_substep":  # the centred fluxes divide by dt
+        assert _first_order(fn, dt=0.0) == -1 and "dt" in _err()
+
+
+def test_kp07_step_order_validation():
+    lib = _lib.lib()
+    p = 0x1000
+    args = [p + 0x100000 * k for k in range(6)]
+
+    def call(step_order):
+        return lib.fvmi_swe_kp07_substep(16, 16, 1.0, 1.0, 0.1, 9.81, 1.3, step_order, 0, args[0], 256, args[1], 256,
+                                         args[2], 256, args[3], 256, args[4], 256, args[5], 256, None, 0, None, None)
+    assert call((2 << 16) | 2) == -1 and "order" in _err()
+    assert call((0 << 16) | 3) == -1
+    assert call((1 << 16) | 1) == -1
+
+
+def test_ensemble_substep_validation():
+    lib = _lib.lib()
+    assert lib.fvmi_ensemble_substep(0, None, None, None, 0, 0, None, None) == -1
+    assert lib.fvmi_ensemble_substep(7, 0x1000, 0x2000, 0x3000, 0, 0, None, None) == -1 and "scheme" in _err()
+    assert lib.fvmi_ensemble_substep(0, 0x1000, 0x2000, 0x3000, 0, 2, None, None) == -1 and "step" in _err()
