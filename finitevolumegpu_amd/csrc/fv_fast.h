@@ -7,7 +7,7 @@
 //   * reciprocals of a pair share one v_rcp of the product;
 //   * quotients with a common denominator reuse one reciprocal.
 // Rounding differs from the exact policy by a few ulp per operation; the
-// results are judged against the float32 oracle at relative L1 <= 1e-5.
+// results are held to relative L1 <= 1e-5 of the float32 oracle.
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -13,7 +13,7 @@
 //              are bitwise equal to an IEEE float32 evaluation of the
 //              reference arithmetic.
 //   MathFast   hardware reciprocal / square root (v_rcp_f32, v_sqrt_f32, about
-//              1 ulp) with shared reciprocals and contraction allowed.  Judged
+//              1 ulp) with shared reciprocals and contraction allowed.  Held
 //              against the float32 oracle at relative L1 <= 1e-5.
 #pragma once
 

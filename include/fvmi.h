@@ -37,7 +37,7 @@ extern "C" {
 #define FVMI_ERR_HIP (-2)
 #define FVMI_ERR_UNSUPPORTED (-3)
 
-#define FVMI_MATH_FAST 0  /* hardware rcp/sqrt, contraction: judged at rel-L1 <= 1e-5 */
+#define FVMI_MATH_FAST 0  /* hardware rcp/sqrt, contraction: within rel-L1 1e-5 of exact */
 #define FVMI_MATH_EXACT 1 /* IEEE div/sqrt, no FMA: bitwise equal to an IEEE float32 evaluation */
 
 #define FVMI_MEMCPY_HTOD 1
