@@ -274,6 +274,8 @@ class Ensemble:
         # zeroed (ghost memory included) and the initial state copied in
         pitch = instance_pitch(nx)
         rows = ny + 4
+        if rows * pitch >= 1 << 31:  # the kernels' buffer descriptors cover 2 GiB per field
+            raise ValueError("instance %d: (ny+4)*pitch must be below 2 GiB" % i)
         handle = self.arena.alloc(2 * self.nvars * rows * pitch)
         base = self.arena.address(handle)
         self.buffers[i] = handle

@@ -77,6 +77,7 @@ __device__ __forceinline__ void ensemble_segment(const EnsembleArgs& e, const in
     a.inv_dy = I.inv_dy;
     a.dtdx = a.dt / a.dx;
     a.dtdy = a.dt / a.dy;
+    set_slope_constants(a);
 
     Strip st;
     setup_strip<S, M>(a, st, lane, __builtin_amdgcn_readfirstlane(seg.y), __builtin_amdgcn_readfirstlane(seg.z),

@@ -20,30 +20,37 @@ __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x);
 __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 // Predictor difference F(QL) - F(QR) of the SWE flux (SWECommon.h:29-37), one
-// reciprocal for both faces.
-__device__ __forceinline__ void swe_pred_diff(const float QL[3], const float QR[3], float hg, float D[3]) {
+// reciprocal for both faces.  The faces are h0 -/+ (theta/2) m0 in depth, so
+// the pressure term g/2 (QL0^2 - QR0^2) = -g theta m0 h0.
+__device__ __forceinline__ void swe_pred_diff(const float QL[3], const float QR[3], float h0, float m0, float g_theta,
+                                              float D[3]) {
     const float r = rcp(QL[0] * QR[0]);
     const float uL = QL[1] * (QR[0] * r);
     const float uR = QR[1] * (QL[0] * r);
     D[0] = QL[1] - QR[1];
-    D[1] = (QL[1] * uL - QR[1] * uR) + hg * (QL[0] * QL[0] - QR[0] * QR[0]);
+    D[1] = (QL[1] * uL - QR[1] * uR) - g_theta * (m0 * h0);
     D[2] = QL[2] * uL - QR[2] * uR;
 }
 
-// Central-upwind flux (SWECommon.h:228-241)
+// Central-upwind flux (SWECommon.h:228-241),
+//   (ap F(qm) - am F(qp) + ap am (qp - qm)) / (ap - am)
+//     = wm (F(qm) - am qm) - wp (F(qp) - ap qp),  wm = ap/(ap-am), wp = am/(ap-am),
+// where F(q) - a q = (q1 - a q0, q1 (u - a) + g/2 q0^2, q2 (u - a)).
 __device__ __forceinline__ void swe_cu_flux(const float qm[3], const float qp[3], float hg, float sg, float out[3]) {
     const float sp = rsq(qp[0]), sm = rsq(qm[0]);
     const float up = qp[1] * (sp * sp), um = qm[1] * (sm * sm);
     const float cp = sg * qp[0] * sp, cm = sg * qm[0] * sm;
-    const float Fp1 = qp[1] * up + hg * qp[0] * qp[0], Fp2 = qp[2] * up;
-    const float Fm1 = qm[1] * um + hg * qm[0] * qm[0], Fm2 = qm[2] * um;
     const float am = __builtin_fminf(__builtin_fminf(um - cm, up - cp), 0.0f);
     const float ap = __builtin_fmaxf(__builtin_fmaxf(um + cm, up + cp), 0.0f);
     const float iden = rcp(ap - am);
-    const float wm = ap * iden, wp = am * iden, wq = ap * am * iden;
-    out[0] = (wm * qm[1] - wp * qp[1]) + wq * (qp[0] - qm[0]);
-    out[1] = (wm * Fm1 - wp * Fp1) + wq * (qp[1] - qm[1]);
-    out[2] = (wm * Fm2 - wp * Fp2) + wq * (qp[2] - qm[2]);
+    const float wm = ap * iden, wp = am * iden;
+    const float dm = um - am, dp = up - ap;
+    const float A0 = qm[1] - am * qm[0], B0 = qp[1] - ap * qp[0];
+    const float A1 = qm[1] * dm + hg * qm[0] * qm[0], B1 = qp[1] * dp + hg * qp[0] * qp[0];
+    const float A2 = qm[2] * dm, B2 = qp[2] * dp;
+    out[0] = wm * A0 - wp * B0;
+    out[1] = wm * A1 - wp * B1;
+    out[2] = wm * A2 - wp * B2;
 }
 
 // HLL flux, SWE (SWECommon.h:274-313)

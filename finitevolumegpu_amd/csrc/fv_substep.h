@@ -64,7 +64,17 @@ struct SubstepArgs {
     int xcd_remap;
     // derived constants for the fast policy (set by the host launcher)
     float hg, sqrt_g, inv_g, inv_dx, inv_dy, dtdx, dtdy;
+    float half_theta;  // theta / 2
+    float inv_2theta;  // 1 / (2 theta), 0 for theta <= 0
+    float g_theta;     // g theta
 };
+
+// the slope constants of the fast policy from g and theta
+__host__ __device__ inline void set_slope_constants(SubstepArgs& a) {
+    a.half_theta = 0.5f * a.theta;
+    a.inv_2theta = a.theta > 0.0f ? 0.5f / a.theta : 0.0f;
+    a.g_theta = a.g * a.theta;
+}
 
 struct AxisMap {
     int idx;
@@ -96,6 +106,60 @@ __device__ __forceinline__ AxisMap map_axis_gc(int k, int n, int bc_lo, int bc_h
 
 __device__ __forceinline__ AxisMap map_axis(int k, int n, int bc_lo, int bc_hi) {
     return map_axis_gc<2>(k, n, bc_lo, bc_hi);
+}
+
+// The same map along y in affine, branch-free form for the row march (the row
+// index is wave-uniform, so the selects stay scalar): a ghost row k < GC reads
+// memory row lo_a*k + lo_b, a ghost row k >= n+GC (clamped to the padded
+// array) hi_a*k + hi_b, an interior row itself.  A mirroring side flips the
+// sign of the y momentum (sign-bit mask).
+struct RowMap {
+    int lo_a, lo_b, hi_a, hi_b;
+    int n_gc;   // n + GC, the first ghost row of the high side
+    int k_max;  // n + 2 GC - 1, the last padded row
+    unsigned lo_sign, hi_sign;
+};
+
+template <int GC>
+__device__ __forceinline__ void row_map_side(int bc, int n, bool hi, int& a, int& b, unsigned& sign) {
+    a = 1;
+    b = 0;
+    sign = 0u;
+    if (bc == BC_PERIODIC) {
+        b = hi ? -n : n;
+    } else if (bc == BC_REFLECTIVE) {
+        a = -1;
+        b = hi ? 2 * (n + GC) - 1 : 2 * GC - 1;
+        sign = 0x80000000u;
+    } else if (bc == BC_OUTFLOW) {
+        a = 0;
+        b = hi ? n + GC - 1 : GC;
+    }
+}
+
+template <int GC>
+__device__ __forceinline__ RowMap make_row_map(int n, int bc_lo, int bc_hi) {
+    RowMap m;
+    row_map_side<GC>(bc_lo, n, false, m.lo_a, m.lo_b, m.lo_sign);
+    row_map_side<GC>(bc_hi, n, true, m.hi_a, m.hi_b, m.hi_sign);
+    m.n_gc = n + GC;
+    m.k_max = n + 2 * GC - 1;
+    return m;
+}
+
+// memory row of padded row k (wave-uniform) and the y-momentum sign mask
+template <int GC>
+__device__ __forceinline__ int map_row(const RowMap& m, int k, unsigned& sign) {
+    const bool lo = k < GC, hi = k >= m.n_gc;
+    // every candidate is computed, then selected (no branches)
+    const int lo_row = m.lo_a * k + m.lo_b;
+    const int hi_row = m.hi_a * min(k, m.k_max) + m.hi_b;
+    int row = hi ? hi_row : k;
+    row = lo ? lo_row : row;
+    unsigned sg = hi ? m.hi_sign : 0u;
+    sg = lo ? m.lo_sign : sg;
+    sign = __builtin_amdgcn_readfirstlane(sg);
+    return __builtin_amdgcn_readfirstlane(row);
 }
 
 // Cross-lane moves: the value held by lane-1 / lane+1, as one DPP row move
@@ -139,15 +203,17 @@ struct SweBase {
     static constexpr int NV = 3;
     static constexpr bool kEuler = false;
     __device__ static void pred_diff(const float QL[3], const float QR[3], const SubstepArgs& a, float D[3]) {
-        if constexpr (M::exact) {
-            float FL[3], FR[3];
-            swe_F<M>(QL, a.g, FL);
-            swe_F<M>(QR, a.g, FR);
+        float FL[3], FR[3];
+        swe_F<M>(QL, a.g, FL);
+        swe_F<M>(QR, a.g, FR);
 #pragma unroll
-            for (int k = 0; k < 3; ++k) D[k] = FL[k] - FR[k];
-        } else {
-            fast::swe_pred_diff(QL, QR, a.hg, D);
-        }
+        for (int k = 0; k < 3; ++k) D[k] = FL[k] - FR[k];
+    }
+    // fast policy: faces QL/QR = c -/+ (theta/2) m of the cell c with limited
+    // slope m (in units of theta)
+    __device__ static void pred_diff_fast(const float QL[3], const float QR[3], const float c[3], const float m[3],
+                                          const SubstepArgs& a, float D[3]) {
+        fast::swe_pred_diff(QL, QR, c[0], m[0], a.g_theta, D);
     }
     __device__ static float cell_dt(const float q[3], const SubstepArgs& a) { return swe_cell_dt<M>(q, a.dx, a.dy, a.g); }
     __device__ static float cell_rate(const float q[3], const SubstepArgs& a) {
@@ -195,15 +261,17 @@ struct EulerKP07 {
     static constexpr bool kEuler = true;
     // The predictor passes gamma in the pressure slot: EE2D_KP07_dimsplit.cu:57-58, 102-103.
     __device__ static void pred_diff(const float QL[4], const float QR[4], const SubstepArgs& a, float D[4]) {
-        if constexpr (M::exact) {
-            float FL[4], FR[4];
-            euler_F<M>(QL, a.gamma, FL);
-            euler_F<M>(QR, a.gamma, FR);
+        float FL[4], FR[4];
+        euler_F<M>(QL, a.gamma, FL);
+        euler_F<M>(QR, a.gamma, FR);
 #pragma unroll
-            for (int k = 0; k < 4; ++k) D[k] = FL[k] - FR[k];
-        } else {
-            fast::euler_pred_diff(QL, QR, a.gamma, D);
-        }
+        for (int k = 0; k < 4; ++k) D[k] = FL[k] - FR[k];
+    }
+    __device__ static void pred_diff_fast(const float QL[4], const float QR[4], const float c[4], const float m[4],
+                                          const SubstepArgs& a, float D[4]) {
+        (void)c;
+        (void)m;
+        fast::euler_pred_diff(QL, QR, a.gamma, D);
     }
     __device__ static void flux_x(const float l[4], const float r[4], const SubstepArgs& a, float o[4]) {
         if constexpr (M::exact)
@@ -245,13 +313,28 @@ __device__ __forceinline__ void reconstruct(const float l[S::NV], const float c[
                                             const SubstepArgs& a, float kk, float qbl[S::NV], float qbr[S::NV]) {
     constexpr int NV = S::NV;
     float QL[NV], QR[NV], D[NV];
+    if constexpr (M::exact) {
 #pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        const float s = minmod_slope<M>(l[k], c[k], r[k], a.theta);
-        QL[k] = c[k] - 0.5f * s;
-        QR[k] = c[k] + 0.5f * s;
+        for (int k = 0; k < NV; ++k) {
+            const float s = minmod_slope<M>(l[k], c[k], r[k], a.theta);
+            QL[k] = c[k] - 0.5f * s;
+            QR[k] = c[k] + 0.5f * s;
+        }
+        S::pred_diff(QL, QR, a, D);
+    } else {
+        // minmod(theta b, (b + f)/2, theta f) = theta minmod(b, (b + f)/(2 theta), f)
+        // for theta > 0, with b, f the backward and forward differences; each
+        // neighbour enters once (a cross-lane operand in the x sweep)
+        float m[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            const float b = c[k] - l[k], f = r[k] - c[k];
+            m[k] = __builtin_amdgcn_fmed3f(__builtin_amdgcn_fmed3f(b, f, 0.0f), (b + f) * a.inv_2theta, 0.0f);
+            QL[k] = __builtin_fmaf(-a.half_theta, m[k], c[k]);
+            QR[k] = __builtin_fmaf(a.half_theta, m[k], c[k]);
+        }
+        S::pred_diff_fast(QL, QR, c, m, a, D);
     }
-    S::pred_diff(QL, QR, a, D);
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
         qbl[k] = QL[k] + kk * D[k];
@@ -309,6 +392,11 @@ __device__ __forceinline__ void x_sweep_row(float q[S::NV], const SubstepArgs& a
 // Raw buffer descriptor of one padded field (dword3 = 32-bit data format,
 // the value used for raw buffers on gfx9).  Reads past `bytes` return 0 and
 // writes past it are dropped by the hardware.
+//
+// Lanes that must not write use the vector offset kMaskedOffset (2 GiB).  The
+// host keeps every field below 2 GiB, so that offset is out of range whether
+// or not the scalar (row) offset, itself below 2 GiB, is added to it.
+constexpr int kMaskedOffset = static_cast<int>(0x80000000u);
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t field_rsrc(const void* base, unsigned bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
 }
@@ -319,10 +407,11 @@ struct Strip {
     int col;       // padded column of this lane
     int row0;      // padded row of r = 0
     int r_end;     // rows r in [0, r_end)
-    int lane_off;  // byte offset of this lane's column in a row (after the boundary map)
-    bool mirror_x;
-    bool lane_out;  // lane writes its column
-    int bcN, bcS;
+    int lane_off;    // byte offset of this lane's column in a row (after the boundary map)
+    unsigned xsign;  // sign-bit mask of the x momentum (mirrored ghost column)
+    bool lane_out;   // lane writes its column
+    int store_off;   // byte offset of the column for writing lanes, kMaskedOffset for the others
+    RowMap ymap;
     __amdgpu_buffer_rsrc_t rin[4], rout[4];
 };
 
@@ -330,28 +419,27 @@ template <int NV>
 __device__ __forceinline__ void load_row(const SubstepArgs& a, const Strip& st, int r, float v[NV]) {
     // Row offsets are wave-uniform (SGPR soffset); the lane adds its column
     // offset (VGPR voffset): no per-lane address arithmetic.
-    const AxisMap my = map_axis(__builtin_amdgcn_readfirstlane(st.row0 + r), a.ny, st.bcS, st.bcN);
-    const int row = __builtin_amdgcn_readfirstlane(my.idx);
+    unsigned ysign;
+    const int row = map_row<2>(st.ymap, __builtin_amdgcn_readfirstlane(st.row0 + r), ysign);
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
         // the b32 buffer builtins move raw 32-bit words
-        const float x =
-            __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(st.rin[k], st.lane_off, row * a.in_pitch[k], 0));
+        unsigned x = __builtin_amdgcn_raw_buffer_load_b32(st.rin[k], st.lane_off, row * a.in_pitch[k], 0);
         // x mirrors flip the x momentum, y mirrors the y momentum
-        const bool flip = (k == 1 && st.mirror_x) || (k == 2 && my.mirrored);
-        v[k] = flip ? -x : x;
+        if (k == 1) x ^= st.xsign;
+        if (k == 2) x ^= ysign;
+        v[k] = __uint_as_float(x);
     }
 }
 
 template <int NV>
 __device__ __forceinline__ void store_row(const SubstepArgs& a, const Strip& st, int r, const float (&q)[NV]) {
     const int prow = __builtin_amdgcn_readfirstlane(st.row0 + r - 2);
-    if (st.lane_out) {
+    // every lane stores; the halo lanes' offset is out of range and the
+    // hardware drops their writes (no branch around the stores)
 #pragma unroll
-        for (int k = 0; k < NV; ++k)
-            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(q[k]), st.rout[k], st.col * (int)sizeof(float),
-                                                  prow * a.out_pitch[k], 0);
-    }
+    for (int k = 0; k < NV; ++k)
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(q[k]), st.rout[k], st.store_off, prow * a.out_pitch[k], 0);
 }
 
 // CFL accumulator: the exact policy keeps min(dt) as the reference's writeCfl
@@ -564,12 +652,12 @@ __device__ __forceinline__ void setup_strip(const SubstepArgs& a, Strip& st, int
     st.col = a.x0 + kStripCols * strip_x + lane;
     st.row0 = __builtin_amdgcn_readfirstlane(a.y0 + row_in);  // padded row of r = 0
     st.r_end = __builtin_amdgcn_readfirstlane(count + 4);     // output rows r = 4 .. count+3
-    st.bcN = (a.bc >> 24) & 0xF;
-    st.bcS = (a.bc >> 16) & 0xF;
+    st.ymap = make_row_map<2>(a.ny, (a.bc >> 16) & 0xF, (a.bc >> 24) & 0xF);
     const AxisMap mx = map_axis(st.col, a.nx, a.bc & 0xF, (a.bc >> 8) & 0xF);
     st.lane_off = mx.idx * (int)sizeof(float);
-    st.mirror_x = mx.mirrored;
+    st.xsign = mx.mirrored ? 0x80000000u : 0u;
     st.lane_out = lane >= 2 && lane < 2 + kStripCols && st.col < a.x1 + 2;
+    st.store_off = st.lane_out ? st.col * (int)sizeof(float) : kMaskedOffset;
 #pragma unroll
     for (int k = 0; k < S::NV; ++k) {
         st.rin[k] = field_rsrc(a.in[k], (unsigned)(a.ny + 4) * (unsigned)a.in_pitch[k]);

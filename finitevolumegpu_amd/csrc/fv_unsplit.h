@@ -47,7 +47,22 @@ struct UnsplitArgs {
     // dy/(2dt), dt/(2dx), dt/(2dy) (FORCE)
     float hg, sqrt_g, inv_dx, inv_dy, dtdx, dtdy;
     float lxf_kx, lxf_ky, lf_kx, lf_ky, lw_kx, lw_ky;
+    float half_theta, inv_2theta;  // theta/2 and 1/(2 theta) (0 for theta <= 0), KP07
 };
+
+// Half the limited slope, s/2, of a cell with neighbours l and r.  Fast form:
+// minmod(theta b, (b + f)/2, theta f) = theta minmod(b, (b + f)/(2 theta), f)
+// for theta > 0 (b, f the backward and forward differences).
+template <class M>
+__device__ __forceinline__ float half_slope(float l, float c, float r, const UnsplitArgs& a) {
+    if constexpr (M::exact) {
+        return 0.5f * minmod_slope<M>(l, c, r, a.theta);
+    } else {
+        const float b = c - l, f = r - c;
+        return a.half_theta *
+               __builtin_amdgcn_fmed3f(__builtin_amdgcn_fmed3f(b, f, 0.0f), (b + f) * a.inv_2theta, 0.0f);
+    }
+}
 
 // ---------------------------------------------------------------------------
 // Centred fluxes (SWECommon.h:404-473), in sweep order (h, normal, tangential)
@@ -162,10 +177,11 @@ struct UStrip {
     int col;       // padded column of this lane
     int row0;      // padded row of r = 0
     int r_end;     // rows r in [0, r_end)
-    int lane_off;  // byte offset of this lane's column (after the boundary map)
-    bool mirror_x;
+    int lane_off;    // byte offset of this lane's column (after the boundary map)
+    unsigned xsign;  // sign-bit mask of the x momentum (mirrored ghost column)
     bool lane_out;
-    int bcN, bcS;
+    int store_off;  // byte offset of the column for writing lanes, kMaskedOffset for the others
+    RowMap ymap;
     __amdgpu_buffer_rsrc_t rin[3], rout[3];
 };
 
@@ -176,12 +192,12 @@ __device__ __forceinline__ void u_setup(const UnsplitArgs& a, UStrip<GC>& st, in
     st.col = W * strip_x + lane;
     st.row0 = __builtin_amdgcn_readfirstlane(row_in);  // padded row of the first row read
     st.r_end = __builtin_amdgcn_readfirstlane(count + 2 * GC);
-    st.bcN = (a.bc >> 24) & 0xF;
-    st.bcS = (a.bc >> 16) & 0xF;
+    st.ymap = make_row_map<GC>(a.ny, (a.bc >> 16) & 0xF, (a.bc >> 24) & 0xF);
     const AxisMap mx = map_axis_gc<GC>(st.col, a.nx, a.bc & 0xF, (a.bc >> 8) & 0xF);
     st.lane_off = mx.idx * (int)sizeof(float);
-    st.mirror_x = mx.mirrored;
+    st.xsign = mx.mirrored ? 0x80000000u : 0u;
     st.lane_out = lane >= GC && lane < GC + W && st.col < a.nx + GC;
+    st.store_off = st.lane_out ? st.col * (int)sizeof(float) : kMaskedOffset;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         st.rin[k] = field_rsrc(a.in[k], (unsigned)(a.ny + 2 * GC) * (unsigned)a.in_pitch[k]);
@@ -191,34 +207,33 @@ __device__ __forceinline__ void u_setup(const UnsplitArgs& a, UStrip<GC>& st, in
 
 template <int GC>
 __device__ __forceinline__ void u_load(const UnsplitArgs& a, const UStrip<GC>& st, int r, float (&v)[3]) {
-    const AxisMap my = map_axis_gc<GC>(__builtin_amdgcn_readfirstlane(st.row0 + r), a.ny, st.bcS, st.bcN);
-    const int row = __builtin_amdgcn_readfirstlane(my.idx);
+    unsigned ysign;
+    const int row = map_row<GC>(st.ymap, __builtin_amdgcn_readfirstlane(st.row0 + r), ysign);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const float x =
-            __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(st.rin[k], st.lane_off, row * a.in_pitch[k], 0));
-        const bool flip = (k == 1 && st.mirror_x) || (k == 2 && my.mirrored);
-        v[k] = flip ? -x : x;
+        unsigned x = __builtin_amdgcn_raw_buffer_load_b32(st.rin[k], st.lane_off, row * a.in_pitch[k], 0);
+        if (k == 1) x ^= st.xsign;
+        if (k == 2) x ^= ysign;
+        v[k] = __uint_as_float(x);
     }
 }
 
 // Output row `prow` (padded): the stored state, or for the second RK2 stage
 // 0.5 (old + q) with `old` the output buffer's value (SWE2D_KP07.cu:205-209).
+// Every lane stores; the halo lanes' offset is out of range (kMaskedOffset), so
+// the hardware drops their writes (and returns 0 for their reads).
 template <int GC>
 __device__ __forceinline__ void u_store(const UnsplitArgs& a, const UStrip<GC>& st, int prow, const float (&q)[3]) {
     const int row = __builtin_amdgcn_readfirstlane(prow);
-    if (st.lane_out) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            float v = q[k];
-            if (a.stage2) {
-                const float old = __uint_as_float(
-                    __builtin_amdgcn_raw_buffer_load_b32(st.rout[k], st.col * (int)sizeof(float), row * a.out_pitch[k], 0));
-                v = 0.5f * (old + q[k]);
-            }
-            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), st.rout[k], st.col * (int)sizeof(float),
-                                                  row * a.out_pitch[k], 0);
+    for (int k = 0; k < 3; ++k) {
+        float v = q[k];
+        if (a.stage2) {
+            const float old =
+                __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(st.rout[k], st.store_off, row * a.out_pitch[k], 0));
+            v = 0.5f * (old + q[k]);
         }
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), st.rout[k], st.store_off, row * a.out_pitch[k], 0);
     }
 }
 
@@ -270,9 +285,9 @@ __device__ __forceinline__ void x_fluxes_kp07(const float (&q)[3], const Unsplit
     float qm[3], qp[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const float s = minmod_slope<M>(from_left(q[k]), q[k], from_right(q[k]), a.theta);
-        qm[k] = q[k] + 0.5f * s;               // right face of cell c
-        qp[k] = from_right(q[k] - 0.5f * s);   // left face of cell c + 1
+        const float hs = half_slope<M>(from_left(q[k]), q[k], from_right(q[k]), a);
+        qm[k] = q[k] + hs;               // right face of cell c
+        qp[k] = from_right(q[k] - hs);   // left face of cell c + 1
     }
     S::flux(qm, qp, a, F);
 }
@@ -331,9 +346,9 @@ __device__ __forceinline__ void kp07u_row(const UnsplitArgs& a, const UStrip<2>&
     float bot[3], top_next[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const float s = minmod_slope<M>(w0[k], w1[k], w2[k], a.theta);
-        top_next[k] = w1[k] + 0.5f * s;
-        bot[k] = w1[k] - 0.5f * s;
+        const float hs = half_slope<M>(w0[k], w1[k], w2[k], a);
+        top_next[k] = w1[k] + hs;
+        bot[k] = w1[k] - hs;
     }
     float G[3];
     {
@@ -407,8 +422,7 @@ __device__ __forceinline__ float march_kp07u(const UnsplitArgs& a, const UStrip<
     for (int m = 0; m < N; ++m) u_load<2>(a, st, min(m, last), ring[m]);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {  // top face of row 1
-        const float s = minmod_slope<M>(ring[0][k], ring[1][k], ring[2][k], a.theta);
-        P[k] = ring[1][k] + 0.5f * s;
+        P[k] = ring[1][k] + half_slope<M>(ring[0][k], ring[1][k], ring[2][k], a);
         G[k] = 0.0f;
     }
     u_load<2>(a, st, min(3 + PF, last), ring[(3 + PF) % N]);

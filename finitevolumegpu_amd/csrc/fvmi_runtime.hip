@@ -98,10 +98,11 @@ int run_substep(int scheme, SubstepArgs& a, int nvars, const fvmi_launch_t* laun
         if (a.in[k] == a.out[k]) return fail(FVMI_ERR_INVALID, "field %d: input and output alias", k);
         if (a.in_pitch[k] < min_pitch || a.out_pitch[k] < min_pitch || (a.in_pitch[k] & 3) || (a.out_pitch[k] & 3))
             return fail(FVMI_ERR_INVALID, "field %d: pitch must be >= %d bytes and a multiple of 4", k, min_pitch);
-        // fields are addressed through 32-bit buffer descriptors
-        if ((size_t)(a.ny + 4) * (size_t)a.in_pitch[k] >= ((size_t)1 << 32) ||
-            (size_t)(a.ny + 4) * (size_t)a.out_pitch[k] >= ((size_t)1 << 32))
-            return fail(FVMI_ERR_INVALID, "field %d: (ny+4)*pitch must be below 4 GiB", k);
+        // fields are addressed through buffer descriptors whose range must stay
+        // below the 2 GiB offset that masked lanes' stores use
+        if ((size_t)(a.ny + 4) * (size_t)a.in_pitch[k] >= ((size_t)1 << 31) ||
+            (size_t)(a.ny + 4) * (size_t)a.out_pitch[k] >= ((size_t)1 << 31))
+            return fail(FVMI_ERR_INVALID, "field %d: (ny+4)*pitch must be below 2 GiB", k);
     }
     if (a.x1 == 0) a.x1 = a.nx;
     if (a.y1 == 0) a.y1 = a.ny;
@@ -118,6 +119,7 @@ int run_substep(int scheme, SubstepArgs& a, int nvars, const fvmi_launch_t* laun
     a.inv_dy = 1.0f / a.dy;
     a.dtdx = a.dt / a.dx;
     a.dtdy = a.dt / a.dy;
+    set_slope_constants(a);
     const int grid = plan_substep(a, current_cu_count());
     hipError_t e = (math == FVMI_MATH_EXACT) ? launch_substep_exact(scheme, a, grid, (hipStream_t)stream)
                                              : launch_substep_fast(scheme, a, grid, (hipStream_t)stream);
@@ -270,9 +272,9 @@ int run_unsplit(int scheme, UnsplitArgs& a, int gc, const fvmi_launch_t* launch,
         if (a.in[k] == a.out[k]) return fail(FVMI_ERR_INVALID, "field %d: input and output alias", k);
         if (a.in_pitch[k] < min_pitch || a.out_pitch[k] < min_pitch || (a.in_pitch[k] & 3) || (a.out_pitch[k] & 3))
             return fail(FVMI_ERR_INVALID, "field %d: pitch must be >= %d bytes and a multiple of 4", k, min_pitch);
-        if ((size_t)(a.ny + 2 * gc) * (size_t)a.in_pitch[k] >= ((size_t)1 << 32) ||
-            (size_t)(a.ny + 2 * gc) * (size_t)a.out_pitch[k] >= ((size_t)1 << 32))
-            return fail(FVMI_ERR_INVALID, "field %d: (ny+%d)*pitch must be below 4 GiB", k, 2 * gc);
+        if ((size_t)(a.ny + 2 * gc) * (size_t)a.in_pitch[k] >= ((size_t)1 << 31) ||
+            (size_t)(a.ny + 2 * gc) * (size_t)a.out_pitch[k] >= ((size_t)1 << 31))
+            return fail(FVMI_ERR_INVALID, "field %d: (ny+%d)*pitch must be below 2 GiB", k, 2 * gc);
     }
     SubstepArgs shape{};
     int math = 0;
@@ -286,6 +288,8 @@ int run_unsplit(int scheme, UnsplitArgs& a, int gc, const fvmi_launch_t* launch,
     a.inv_dy = 1.0f / a.dy;
     a.dtdx = a.dt / a.dx;
     a.dtdy = a.dt / a.dy;
+    a.half_theta = 0.5f * a.theta;
+    a.inv_2theta = a.theta > 0.0f ? 0.5f / a.theta : 0.0f;
     if (centred) {
         a.lxf_kx = a.dx / (4.0f * a.dt);
         a.lxf_ky = a.dy / (4.0f * a.dt);

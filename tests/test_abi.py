@@ -75,6 +75,14 @@ def test_validation_rejects_bad_step_dt_pitch():
     assert "pitch" in _err()
 
 
+def test_validation_rejects_fields_of_2gib():
+    # masked lanes store at a 2 GiB offset, so every field must stay below it
+    assert _substep(pitch=((1 << 31) // 20 + 4) & ~3) == -1  # (16 + 4) rows
+    assert "2 GiB" in _err()
+    assert _first_order("fvmi_swe_lxf_substep", pitch=((1 << 31) // 18 + 4) & ~3) == -1  # (16 + 2) rows
+    assert "2 GiB" in _err()
+
+
 def test_validation_rejects_aliasing():
     lib = _lib.lib()
     p = 0x1000
