@@ -10,7 +10,10 @@ before returning the host array.
 """
 
 import ctypes
+import json
 import logging
+import os
+import subprocess
 import time
 
 import numpy as np
@@ -208,3 +211,292 @@ class ArakawaA2D:
 
 # Drop-in name for code written against the reference.
 CudaArray2D = HipArray2D
+
+
+# ---------------------------------------------------------------------------
+# Simulation output (Common.py:48-208, 361-431)
+# ---------------------------------------------------------------------------
+
+
+def safeCall(cmd):
+    """Output of ``cmd`` run in this package's folder (Common.py:48-66).  Unlike
+    the reference, a failing command (no git, no repository) returns "" and
+    logs a warning instead of raising, so output files can be written from an
+    installed tree."""
+    logger = logging.getLogger(__name__)
+    try:
+        return subprocess.check_output(cmd, stderr=subprocess.STDOUT, cwd=os.path.dirname(os.path.realpath(__file__)),
+                                       universal_newlines=True, shell=False)
+    except (OSError, subprocess.CalledProcessError) as e:
+        logger.warning("%s failed: %s", " ".join(cmd), e)
+        return ""
+
+
+def getGitHash():
+    return safeCall(["git", "rev-parse", "HEAD"])
+
+
+def getGitStatus():
+    return safeCall(["git", "status", "--porcelain", "-uno"])
+
+
+def toJson(in_dict, compressed=True):
+    """JSON text of a dict: arrays as lists, anything else JSON cannot encode as
+    its str() (Common.py:74-90)."""
+    logger = logging.getLogger(__name__)
+    out = dict(in_dict)
+    for key, value in out.items():
+        if isinstance(value, np.ndarray):
+            out[key] = value.tolist()
+        else:
+            try:
+                json.dumps(value)
+            except (TypeError, ValueError):
+                out[key] = str(value)
+                logger.warning("JSON: Converting %s to string (%s)", key, out[key])
+    return json.dumps(out)
+
+
+def quantize(data, least_significant_digit):
+    """The rounding netCDF4-python applies to a variable created with
+    ``least_significant_digit=d`` (its ``_quantize``; the reference's
+    environment files do not pin a version): keep
+    ``bits = ceil(log2(10**-e))`` binary digits after the point, with e =
+    floor(log10(10**-d)) (ceil for d < 0), i.e. ``around(2**bits * x) / 2**bits``
+    -- for d = 3, multiples of 1/1024.  float32 input stays float32 (the scaling
+    by a power of two is exact)."""
+    data = np.asarray(data)
+    exp = np.log10(10.0 ** -least_significant_digit)
+    exp = int(np.floor(exp)) if exp < 0 else int(np.ceil(exp))
+    scale = 2.0 ** np.ceil(np.log2(10.0 ** -exp))
+    if data.dtype == np.float32:
+        scale = np.float32(scale)
+    return np.around(scale * data) / scale
+
+
+class _Netcdf3Variable:
+    """One variable of a :class:`Netcdf3Dataset`; writes are rounded to the
+    variable's ``least_significant_digit`` as netCDF4-python rounds them."""
+
+    def __init__(self, var, least_significant_digit=None):
+        self._var = var
+        self._lsd = least_significant_digit
+
+    def __setitem__(self, index, value):
+        value = np.asarray(value)
+        if self._lsd is not None:
+            value = quantize(value.astype(self._var.data.dtype.newbyteorder("="), copy=False), self._lsd)
+        self._var[index] = value
+
+    def __getitem__(self, index):
+        return self._var[index]
+
+    @property
+    def shape(self):
+        return self._var.shape
+
+    @property
+    def dimensions(self):
+        return self._var.dimensions
+
+    def __getattr__(self, name):
+        return getattr(self._var, name)
+
+
+class Netcdf3Dataset:
+    """The part of ``netCDF4.Dataset`` that the output path uses
+    (createDimension, createVariable with zlib / least_significant_digit,
+    variables, dimensions, global attributes, close) over scipy's netCDF-3
+    writer, for environments without netCDF4: a 64-bit-offset netCDF-3 file
+    that netCDF4 and scipy both read.  netCDF-3 has no compression, so ``zlib``
+    is accepted and ignored; ``least_significant_digit`` rounds the data and is
+    recorded as the variable attribute netCDF4 writes."""
+
+    def __init__(self, filename, mode="r", clobber=True, format=None, **kwargs):
+        from scipy.io import netcdf_file
+        mode = (mode or "r")[0]
+        if mode == "w" and not clobber and os.path.exists(filename):
+            raise FileExistsError(filename)
+        object.__setattr__(self, "_f", netcdf_file(filename, mode, mmap=False, version=2))
+        object.__setattr__(self, "_vars", {})
+        object.__setattr__(self, "filepath", lambda: filename)
+
+    def createDimension(self, name, size):
+        self._f.createDimension(name, size)
+
+    def createVariable(self, name, datatype, dimensions=(), zlib=False, least_significant_digit=None, **kwargs):
+        dims = (dimensions,) if isinstance(dimensions, str) else tuple(dimensions)
+        var = self._f.createVariable(name, np.dtype(datatype).char, dims)
+        if least_significant_digit is not None:
+            var.least_significant_digit = np.int32(least_significant_digit)
+        wrapped = _Netcdf3Variable(var, least_significant_digit)
+        self._vars[name] = wrapped
+        return wrapped
+
+    @property
+    def variables(self):
+        out = {name: _Netcdf3Variable(v) for name, v in self._f.variables.items()}
+        out.update(self._vars)
+        return out
+
+    @property
+    def dimensions(self):
+        return self._f.dimensions
+
+    def ncattrs(self):
+        return list(self._f._attributes)
+
+    def getncattr(self, name):
+        return self._f._attributes[name]
+
+    def __setattr__(self, name, value):
+        setattr(self._f, name, value)  # a global attribute
+
+    def __getattr__(self, name):
+        attrs = self.__dict__["_f"]._attributes
+        if name in attrs:
+            return attrs[name]
+        raise AttributeError(name)
+
+    def close(self):
+        self._f.close()
+
+
+def open_dataset(filename, *args, **kwargs):
+    """``netCDF4.Dataset(filename, ...)`` where netCDF4 is installed, else a
+    :class:`Netcdf3Dataset`."""
+    try:
+        import netCDF4
+    except ImportError:
+        return Netcdf3Dataset(filename, *args, **kwargs)
+    return netCDF4.Dataset(filename, *args, **kwargs)
+
+
+class DataDumper:
+    """Opens and closes a netCDF dataset around a ``with`` block (Common.py:361-416).
+
+        with DataDumper("out.nc", mode="w") as data:
+            data.ncfile.createDimension(...)
+
+    In a writing mode an existing file is never overwritten: the name gets a
+    ``_0000``, ``_0001``, ... suffix.  Missing folders are created."""
+
+    def __init__(self, filename, *args, **kwargs):
+        self.logger = logging.getLogger(__name__)
+        filename = os.path.abspath(filename)
+        dirname = os.path.dirname(filename)
+        if dirname and not os.path.isdir(dirname):
+            self.logger.info("Creating directory " + dirname)
+            os.makedirs(dirname)
+        mode = args[0] if args else kwargs.get("mode")
+        if mode and ("w" in mode or "+" in mode or "a" in mode):
+            stem, ext = os.path.splitext(filename)
+            i = 0
+            while os.path.isfile(filename):
+                filename = "{:s}_{:04d}{:s}".format(stem, i, ext)
+                i += 1
+        self.filename = os.path.abspath(filename)
+        self.args = args
+        self.kwargs = kwargs
+        self.logger.info("Initialized " + self.filename)
+
+    def __enter__(self):
+        self.logger.info("Opening " + self.filename)
+        self.ncfile = open_dataset(self.filename, *self.args, **self.kwargs)
+        return self
+
+    def __exit__(self, *args):
+        self.logger.info("Closing " + self.filename)
+        self.ncfile.close()
+
+
+# state fields are left out of the sim_args attribute (too large)
+_FIELD_ARGS = ("rho", "rho_u", "rho_v", "E", "h0", "hu0", "hv0")
+
+
+def runSimulation(simulator, simulator_args, outfile, save_times, save_var_names=[], dt=None):
+    """Runs ``simulator(**simulator_args)`` and stores the variables named in
+    ``save_var_names`` (None entries are skipped, keeping the indices) at every
+    time in ``save_times`` in a netCDF file (Common.py:92-208): dimensions
+    (time, y, x), float32 coordinate variables (x and y from ``getExtent()``,
+    as the reference reads it), float32 ``(time, y, x)`` variables with zlib and
+    ``least_significant_digit=3``, and the global attributes created, git_hash,
+    git_status, simulator and sim_args (JSON).
+
+    Returns ``(filename, profiling_data_sim_runner, profiling_data_mpi)``; the
+    last is the simulator's ``profiling_data_mpi`` (None for a simulator
+    without one, where the reference raises AttributeError).  The field
+    arguments of both the Euler and the shallow-water simulators are left out
+    of sim_args (the reference deletes the four Euler fields)."""
+    logger = logging.getLogger(__name__)
+    prof = {"start": {}, "end": {}}
+    for key in ("t_sim_init", "t_nc_write", "t_full_step"):
+        prof["start"][key] = 0
+        prof["end"][key] = 0
+    prof["start"]["t_sim_init"] = time.time()
+    save_times = np.asarray(save_times)
+    assert len(save_times) > 0, "Need to specify which times to save"
+
+    with Timer("construct") as t:
+        sim = simulator(**simulator_args)
+    logger.info("Constructed in " + str(t.secs) + " seconds")
+
+    with DataDumper(outfile, mode="w", clobber=False) as outdata:
+        nc = outdata.ncfile
+        nc.created = time.ctime(time.time())
+        nc.git_hash = getGitHash()
+        nc.git_status = getGitStatus()
+        nc.simulator = str(simulator)
+        args_for_nc = {k: v for k, v in simulator_args.items() if k not in _FIELD_ARGS}
+        nc.sim_args = toJson(args_for_nc)
+
+        nx, ny = int(simulator_args["nx"]), int(simulator_args["ny"])
+        nc.createDimension("time", len(save_times))
+        nc.createDimension("x", nx)
+        nc.createDimension("y", ny)
+        ncvars = {
+            "time": nc.createVariable("time", np.dtype("float32").char, "time"),
+            "x": nc.createVariable("x", np.dtype("float32").char, "x"),
+            "y": nc.createVariable("y", np.dtype("float32").char, "y"),
+        }
+        ncvars["time"][:] = save_times
+        extent = sim.getExtent()
+        ncvars["x"][:] = np.linspace(extent[0], extent[1], nx)
+        ncvars["y"][:] = np.linspace(extent[2], extent[3], ny)
+
+        download_vars = [i for i, name in enumerate(save_var_names) if name is not None]
+        names = [save_var_names[i] for i in download_vars]
+        for name in names:
+            ncvars[name] = nc.createVariable(name, np.dtype("float32").char, ("time", "y", "x"), zlib=True,
+                                             least_significant_digit=3)
+
+        t_steps = np.empty_like(save_times)
+        t_steps[0] = save_times[0]
+        t_steps[1:] = save_times[1:] - save_times[:-1]
+        prof["end"]["t_sim_init"] = time.time()
+
+        printer = ProgressPrinter(save_times[-1], print_every=10)
+        t_end = 0.0
+        for k in range(len(save_times)):
+            t_step, t_end = t_steps[k], save_times[k]
+            try:
+                sim.check()
+            except AssertionError as e:
+                logger.error("Error after %d steps (t=%f: %s", sim.simSteps(), sim.simTime(), str(e))
+                return outdata.filename, prof, getattr(sim, "profiling_data_mpi", None)
+            prof["start"]["t_full_step"] += time.time()
+            if t_step > 0.0:
+                sim.simulate(t_step, dt)
+            prof["end"]["t_full_step"] += time.time()
+            prof["start"]["t_nc_write"] += time.time()
+            values = sim.download(download_vars)
+            for i, name in enumerate(names):
+                ncvars[name][k, :] = values[i]
+            prof["end"]["t_nc_write"] += time.time()
+            line = printer.getPrintString(t_end)
+            if line:
+                logger.debug(line)
+        if sim.simSteps() > 0:
+            logger.debug("Simulated to t=%f in %d timesteps (average dt=%f)", t_end, sim.simSteps(),
+                         sim.simTime() / sim.simSteps())
+    return outdata.filename, prof, getattr(sim, "profiling_data_mpi", None)
