@@ -38,43 +38,60 @@ struct EnsembleArgs {
     int xcd_remap;
 };
 
+// The table is read with vector loads (the kernel also writes it, so the
+// compiler cannot use scalar loads); every value read from it is made
+// wave-uniform explicitly, so the fields' buffer descriptors live in scalar
+// registers (a descriptor in vector registers costs a loop over the distinct
+// values around every buffer access).
+__device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float uniform_f(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ float* uniform_ptr(float* p) {
+    const unsigned long long v = reinterpret_cast<unsigned long long>(p);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
+    return reinterpret_cast<float*>(((unsigned long long)hi << 32) | lo);
+}
+
 // One segment of one instance: the single-grid march with the instance's
 // parameters; returns after the CFL update (step 1).
 template <class S, class M>
 __device__ __forceinline__ void ensemble_segment(const EnsembleArgs& e, const int4 seg, int lane) {
-    const int id = __builtin_amdgcn_readfirstlane(seg.x);
+    const int id = uniform_i(seg.x);
     const fvmi_instance_t& I = e.table[id];
-    if (!I.active) return;
+    if (!uniform_i(I.active)) return;
 
     SubstepArgs a;
-    a.nx = I.nx;
-    a.ny = I.ny;
-    a.dx = I.dx;
-    a.dy = I.dy;
-    a.dt = I.dt_step * 0.5f;  // substep length, KP07_dimsplit.py:108-109
-    a.g = I.g;
-    a.theta = I.theta;
-    a.gamma = I.gamma;
+    a.nx = uniform_i(I.nx);
+    a.ny = uniform_i(I.ny);
+    a.dx = uniform_f(I.dx);
+    a.dy = uniform_f(I.dy);
+    a.dt = uniform_f(I.dt_step) * 0.5f;  // substep length, KP07_dimsplit.py:108-109
+    a.g = uniform_f(I.g);
+    a.theta = uniform_f(I.theta);
+    a.gamma = uniform_f(I.gamma);
     a.step = e.step;
-    a.bc = I.bc;
+    a.bc = uniform_i(I.bc);
     a.x0 = 0;
     a.y0 = 0;
-    a.x1 = I.nx;
-    a.y1 = I.ny;
+    a.x1 = a.nx;
+    a.y1 = a.ny;
+    const int pitch = uniform_i(I.pitch);
 #pragma unroll
     for (int k = 0; k < S::NV; ++k) {
         // after a full step the state is back in buffer 0
-        a.in[k] = I.field[e.step][k];
-        a.out[k] = I.field[e.step ^ 1][k];
-        a.in_pitch[k] = I.pitch;
-        a.out_pitch[k] = I.pitch;
+        a.in[k] = uniform_ptr(I.field[e.step][k]);
+        a.out[k] = uniform_ptr(I.field[e.step ^ 1][k]);
+        a.in_pitch[k] = pitch;
+        a.out_pitch[k] = pitch;
     }
     a.cfl = nullptr;
-    a.hg = 0.5f * I.g;
-    a.sqrt_g = I.sqrt_g;
-    a.inv_g = I.inv_g;
-    a.inv_dx = I.inv_dx;
-    a.inv_dy = I.inv_dy;
+    a.hg = 0.5f * a.g;
+    a.sqrt_g = uniform_f(I.sqrt_g);
+    a.inv_g = uniform_f(I.inv_g);
+    a.inv_dx = uniform_f(I.inv_dx);
+    a.inv_dy = uniform_f(I.inv_dy);
     a.dtdx = a.dt / a.dx;
     a.dtdy = a.dt / a.dy;
     set_slope_constants(a);

@@ -1,6 +1,6 @@
 // Fast arithmetic instantiation (hardware reciprocal / square root, contraction
-// allowed).  The production path; KP07_dimsplit and HLL2 live in
-// fv_kernels_fast_ilp.hip.
+// allowed).  The production path; KP07_dimsplit and HLL2 (single grid and
+// ensemble) live in fv_kernels_fast_ilp.hip.
 #include "fv_launch.h"
 #include "fv_unsplit.h"
 
@@ -19,7 +19,17 @@ hipError_t launch_substep_fast(int scheme, const SubstepArgs& a, int grid, hipSt
     }
 }
 hipError_t launch_ensemble_fast(int scheme, const EnsembleArgs& e, int grid, hipStream_t stream) {
-    return launch_ensemble_substep<MathFast>(scheme, e, grid, stream);
+    switch (scheme) {
+        case SCHEME_KP07:
+        case SCHEME_HLL2:
+            return launch_ensemble_fast_ilp(scheme, e, grid, stream);
+        case SCHEME_WAF:
+            return launch_ensemble_scheme<SweWAF<MathFast>, MathFast>(e, grid, stream);
+        case SCHEME_EULER:
+            return launch_ensemble_scheme<EulerKP07<MathFast>, MathFast>(e, grid, stream);
+        default:
+            return hipErrorInvalidValue;
+    }
 }
 hipError_t launch_unsplit_fast(int scheme, const UnsplitArgs& a, int grid, hipStream_t stream) {
     return launch_unsplit<MathFast>(scheme, a, grid, stream);

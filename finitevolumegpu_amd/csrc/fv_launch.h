@@ -65,6 +65,12 @@ hipError_t launch_substep(int scheme, SubstepArgs a, int grid, hipStream_t strea
     return hipGetLastError();
 }
 
+template <class S, class M>
+hipError_t launch_ensemble_scheme(const EnsembleArgs& e, int grid, hipStream_t stream) {
+    hipLaunchKernelGGL((ensemble_substep_kernel<S, M>), dim3(grid), dim3(kLanes * e.waves_per_block), 0, stream, e);
+    return hipGetLastError();
+}
+
 template <class M>
 hipError_t launch_ensemble_substep(int scheme, EnsembleArgs e, int grid, hipStream_t stream) {
     const dim3 block(kLanes * e.waves_per_block);
@@ -92,8 +98,9 @@ hipError_t launch_ensemble_exact(int scheme, const EnsembleArgs& e, int grid, hi
 hipError_t launch_ensemble_fast(int scheme, const EnsembleArgs& e, int grid, hipStream_t stream);
 hipError_t launch_substep_exact(int scheme, const SubstepArgs& a, int grid, hipStream_t stream);
 hipError_t launch_substep_fast(int scheme, const SubstepArgs& a, int grid, hipStream_t stream);
-// KP07_dimsplit and HLL2 in fast arithmetic (fv_kernels_fast_ilp.hip, built
-// with the ILP-maximising scheduler)
+// KP07_dimsplit and HLL2 in fast arithmetic, single grid and ensemble
+// (fv_kernels_fast_ilp.hip, built with the ILP-maximising scheduler)
 hipError_t launch_substep_fast_ilp(int scheme, const SubstepArgs& a, int grid, hipStream_t stream);
+hipError_t launch_ensemble_fast_ilp(int scheme, const EnsembleArgs& e, int grid, hipStream_t stream);
 
 }  // namespace fvmi
