@@ -16,7 +16,8 @@
  *   - Field buffers are row-major float32 of (ny+4) rows x pitch bytes with
  *     the interior at rows 2..ny+1, columns 2..nx+1 (2 ghost cells per side),
  *     the layout of CudaArray2D (Common.py:502-531).  Pitches are in bytes, as
- *     the reference passes `strides[0]`.
+ *     the reference passes `strides[0]`.  Each field must be below 2 GiB,
+ *     (ny+4)*pitch < 2^31 ((ny+2)*pitch for the one-ghost-cell schemes).
  *   - Boundary conditions are packed N<<24 | S<<16 | E<<8 | W with
  *     0 = Dirichlet, 2 = Periodic, 3 = Reflective (Simulator.py:44-92,
  *     common.h:109-130) and 4 = Outflow (zero-gradient ghost copy; an
