@@ -115,3 +115,33 @@ def test_exact_bitwise_edge_shapes(ctx, scheme, shape):
         sim, ora = _run_pair(ctx, scheme, nx, ny, BCS[bcname], steps=3, math="exact", tile_rows=1)
         for k, (a, b) in enumerate(zip(sim.download(), ora.download())):
             assert np.array_equal(a, b), (scheme, shape, bcname, k, float(np.abs(a - b).max()))
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+@pytest.mark.parametrize("shape", [(59, 9), (60, 5), (61, 7), (131, 11)])
+def test_ghost_frames_never_written(ctx, scheme, shape):
+    """The kernels write interior cells only (writeBlock, common.h:301-318): the
+    halo lanes store at an out-of-range offset, which the hardware drops.  After
+    several steps the ghost frame of both buffers still holds what was put
+    there, at widths that end a strip partly filled."""
+    nx, ny = shape
+    print("case ghost frame", scheme, shape, flush=True)
+    fields = state(scheme, nx, ny, 9)
+    padded = ghost_frame(fields, len(fields))
+    for bcname in ("reflective", "mixed"):
+        sim = make_gpu_sim(ctx, scheme, fields, nx, ny, 1.0, 1.0, BCS[bcname], math="fast", tile_rows=4)
+        for k, a in enumerate(padded):
+            for buf in (sim.u0, sim.u1):
+                buf[k].upload(sim.stream, a, extent=[0, 0, nx + 4, ny + 4])
+        sim.synchronize()
+        dt = np.float32(0.3 * sim.computeDt())
+        for _ in range(3):
+            sim.step(dt)
+        sim.synchronize()
+        frame = np.ones((ny + 4, nx + 4), dtype=bool)
+        frame[2:-2, 2:-2] = False
+        for buf in (sim.u0, sim.u1):
+            for k in range(len(fields)):
+                full = buf[k].download(sim.stream, extent=[0, 0, nx + 4, ny + 4])
+                assert np.array_equal(full[frame], padded[k][frame]), (scheme, shape, bcname, k)
+                assert not np.array_equal(full[2:-2, 2:-2], padded[k][2:-2, 2:-2]) or k > 0

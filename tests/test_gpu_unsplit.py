@@ -167,3 +167,32 @@ def test_exact_bitwise_edge_shapes(ctx, scheme, shape):
         sim, ora = _pair(ctx, scheme, nx, ny, BCS[bcname], 3, "exact", tile_rows=1)
         for k, (a, b) in enumerate(zip(sim.download(), ora.download())):
             assert np.array_equal(a, b), (scheme, shape, bcname, k, float(np.abs(a - b).max()))
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+@pytest.mark.parametrize("shape", [(61, 9), (62, 5), (63, 7), (125, 11)])
+def test_ghost_frames_never_written(ctx, scheme, shape):
+    """Only interior cells are written; the halo lanes' stores (and the second
+    RK2 stage's reads) use an out-of-range offset.  The ghost frame of both
+    buffers keeps its contents, at widths that end a strip partly filled."""
+    nx, ny = shape
+    print("case ghost frame", scheme, shape, flush=True)
+    gc = U.SCHEME_GC[scheme]
+    fields = swe_random_state(nx, ny, 9)
+    padded = _frame(fields, gc)
+    for bcname in ("reflective", "mixed"):
+        sim = _gpu(ctx, scheme, fields, nx, ny, BCS[bcname], "fast", tile_rows=4)
+        for k, a in enumerate(padded):
+            for buf in (sim.u0, sim.u1):
+                buf[k].upload(sim.stream, a, extent=[0, 0, nx + 2 * gc, ny + 2 * gc])
+        sim.synchronize()
+        dt = np.float32(0.3 * sim.computeDt() / sim.dt_factor)
+        for _ in range(3):
+            sim.step(dt)
+        sim.synchronize()
+        frame = np.ones((ny + 2 * gc, nx + 2 * gc), dtype=bool)
+        frame[gc:-gc, gc:-gc] = False
+        for buf in (sim.u0, sim.u1):
+            for k in range(3):
+                full = buf[k].download(sim.stream, extent=[0, 0, nx + 2 * gc, ny + 2 * gc])
+                assert np.array_equal(full[frame], padded[k][frame]), (scheme, shape, bcname, k)
