@@ -39,23 +39,21 @@ struct MathFast {
 // Limiters: GPUSimulators/cuda/limiters.h, common.h
 // ---------------------------------------------------------------------------
 
-// minmodSlope, limiters.h:31-41
+// minmodSlope, limiters.h:31-41, in the reference's operation order (exact
+// policy).  The fast policy evaluates the same minmod as
+// med3(med3(b, f, 0), c, 0) -- the inner median is 0 unless b and f share a
+// sign, and then c shares it too -- in its scaled form (reconstruct in
+// fv_substep.h, half_slope in fv_unsplit.h).
 template <class M>
 __device__ __forceinline__ float minmod_slope(float left, float center, float right, float theta) {
+    static_assert(M::exact, "the fast policy uses the scaled med3 form");
     const float backward = (center - left) * theta;
     const float central = (right - left) * 0.5f;
     const float forward = (right - center) * theta;
-    if constexpr (M::exact) {
-        const float sb = __builtin_copysignf(1.0f, backward);
-        const float sc = __builtin_copysignf(1.0f, central);
-        const float sf = __builtin_copysignf(1.0f, forward);
-        return 0.25f * sb * (sb + sc) * (sc + sf) *
-               fminf(fminf(fabsf(backward), fabsf(central)), fabsf(forward));
-    } else {
-        // Same value: minmod(b, c, f) = med3(med3(b, f, 0), c, 0) -- the inner
-        // median is 0 unless b and f share a sign, and then c shares it too.
-        return __builtin_amdgcn_fmed3f(__builtin_amdgcn_fmed3f(backward, forward, 0.0f), central, 0.0f);
-    }
+    const float sb = __builtin_copysignf(1.0f, backward);
+    const float sc = __builtin_copysignf(1.0f, central);
+    const float sf = __builtin_copysignf(1.0f, forward);
+    return 0.25f * sb * (sb + sc) * (sc + sf) * fminf(fminf(fabsf(backward), fabsf(central)), fabsf(forward));
 }
 
 // generalized_minmod, limiters.h:100-102
@@ -295,17 +293,18 @@ __device__ __forceinline__ void waf_flux_core(const WafCell& L, const WafCell& R
         // shared reciprocal of (S_l - S_star)(S_r - S_star)
         const float el = S_l - S_star, er = S_r - S_star;
         const float ie = M::rcp(el * er);
-        const float s_l = h_l * (S_l - u_l) * (er * ie);
-        const float s_r = h_r * (S_r - u_r) * (el * ie);
+        const float s_l = -dl * (er * ie);  // h_l (S_l - u_l) / (S_l - S_star)
+        const float s_r = -dr * (el * ie);  // h_r (S_r - u_r) / (S_r - S_star)
         const float F_1[3] = {L.hu, L.hu * u_l + hg * h_l * h_l, L.hv * u_l};
         const float F_4[3] = {R.hu, R.hu * u_r + hg * h_r * h_r, R.hv * u_r};
         const float Qsl[3] = {s_l, s_l * S_star, s_l * v_l};
         const float Qsr[3] = {s_r, s_r * S_star, s_r * v_r};
         const float c_1 = S_l * dtd, c_2 = S_star * dtd, c_3 = S_r * dtd;
         const float eps = 1.0e-6f, inv_2eps = 0.5e6f;
+        // desingularize: the sign of x on the magnitude max(|x|, min(x^2/(2 eps) + eps/2, eps))
         auto desing = [&](float x) {
-            return __builtin_copysignf(1.0f, x) *
-                   __builtin_fmaxf(__builtin_fabsf(x), __builtin_fminf(x * x * inv_2eps + 0.5f * eps, eps));
+            return __builtin_copysignf(
+                __builtin_fmaxf(__builtin_fabsf(x), __builtin_fminf(x * x * inv_2eps + 0.5f * eps, eps)), x);
         };
         auto gmm = [](float r) {
             return __builtin_fmaxf(0.0f, __builtin_fminf(1.9f * r, __builtin_fminf((1.0f + r) * 0.5f, 1.9f)));
@@ -319,11 +318,15 @@ __device__ __forceinline__ void waf_flux_core(const WafCell& L, const WafCell& R
         const float A_1 = __builtin_copysignf(1.0f, c_1) * (1.0f - (1.0f - __builtin_fabsf(c_1)) * gmm(r_1));
         const float A_2 = __builtin_copysignf(1.0f, c_2) * (1.0f - (1.0f - __builtin_fabsf(c_2)) * gmm(r_2));
         const float A_3 = __builtin_copysignf(1.0f, c_3) * (1.0f - (1.0f - __builtin_fabsf(c_3)) * gmm(r_3));
+        // With a = F_2 - F_1 = S_l (Q*_l - Q_l1) and b = F_3 - F_4 = S_r (Q*_r - Q_r1),
+        // A_1 (F_2 - F_1) + A_2 (F_3 - F_2) + A_3 (F_4 - F_3)
+        //   = (A_1 - A_2) a + A_2 (F_4 - F_1) + (A_2 - A_3) b.
+        const float w12 = 0.5f * (A_1 - A_2), w2 = 0.5f * A_2, w23 = 0.5f * (A_2 - A_3);
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const float F_2 = F_1[k] + S_l * (Qsl[k] - q_l1[k]);
-            const float F_3 = F_4[k] + S_r * (Qsr[k] - q_r1[k]);
-            out[k] = 0.5f * (F_1[k] + F_4[k]) - 0.5f * (A_1 * (F_2 - F_1[k]) + A_2 * (F_3 - F_2) + A_3 * (F_4[k] - F_3));
+            const float a = S_l * (Qsl[k] - q_l1[k]);
+            const float b = S_r * (Qsr[k] - q_r1[k]);
+            out[k] = 0.5f * (F_1[k] + F_4[k]) - (w12 * a + w2 * (F_4[k] - F_1[k]) + w23 * b);
         }
     }
 }

@@ -219,22 +219,28 @@ __device__ __forceinline__ void u_load(const UnsplitArgs& a, const UStrip<GC>& s
 }
 
 // Output row `prow` (padded): the stored state, or for the second RK2 stage
-// 0.5 (old + q) with `old` the output buffer's value (SWE2D_KP07.cu:205-209).
-// Every lane stores; the halo lanes' offset is out of range (kMaskedOffset), so
-// the hardware drops their writes (and returns 0 for their reads).
-template <int GC>
-__device__ __forceinline__ void u_store(const UnsplitArgs& a, const UStrip<GC>& st, int prow, const float (&q)[3]) {
+// 0.5 (old + q) with `old` the output buffer's value (SWE2D_KP07.cu:205-209),
+// read ahead by u_load_old.  Every lane stores; the halo lanes' offset is out
+// of range (kMaskedOffset), so the hardware drops their writes (and returns 0
+// for their reads).
+template <int GC, bool Avg = false>
+__device__ __forceinline__ void u_store(const UnsplitArgs& a, const UStrip<GC>& st, int prow, const float (&q)[3],
+                                        const float (&old)[3]) {
     const int row = __builtin_amdgcn_readfirstlane(prow);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        float v = q[k];
-        if (a.stage2) {
-            const float old =
-                __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(st.rout[k], st.store_off, row * a.out_pitch[k], 0));
-            v = 0.5f * (old + q[k]);
-        }
+        const float v = Avg ? 0.5f * (old[k] + q[k]) : q[k];
         __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), st.rout[k], st.store_off, row * a.out_pitch[k], 0);
     }
+}
+
+// the output buffer's row `prow` (padded), for the second RK2 stage
+template <int GC>
+__device__ __forceinline__ void u_load_old(const UnsplitArgs& a, const UStrip<GC>& st, int prow, float (&old)[3]) {
+    const int row = __builtin_amdgcn_readfirstlane(prow);
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        old[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(st.rout[k], st.store_off, row * a.out_pitch[k], 0));
 }
 
 template <class S, class M>
@@ -329,7 +335,7 @@ __device__ __forceinline__ void gc1_row(const UnsplitArgs& a, const UStrip<1>& s
 #pragma unroll
             for (int k = 0; k < 3; ++k) out[k] = u_update<M>(w0[k], from_left(F[k]), F[k], g[k], G[k], a);
         }
-        u_store<1>(a, st, st.row0 + r - 1, out);
+        u_store<1>(a, st, st.row0 + r - 1, out, out);
         u_cfl<S, M>(out, a, st.lane_out, acc);
     }
 #pragma unroll
@@ -338,11 +344,12 @@ __device__ __forceinline__ void gc1_row(const UnsplitArgs& a, const UStrip<1>& s
 
 // One step of the KP07 march: w0, w1, w2 = raw rows r-2, r-1, r; top = top
 // face of row r-2 on entry (of r-1 on return); g = flux (r-3, r-2) on entry,
-// (r-2, r-1) on return; emits row r-2.
-template <class S, class M>
+// (r-2, r-1) on return; emits row r-2 (Stage2: averaged with `old`, the output
+// buffer's row r-2).
+template <class S, class M, bool Stage2>
 __device__ __forceinline__ void kp07u_row(const UnsplitArgs& a, const UStrip<2>& st, const float (&w0)[3],
-                                          const float (&w1)[3], const float (&w2)[3], float (&top)[3], float (&g)[3],
-                                          int r, bool emit, float& acc) {
+                                          const float (&w1)[3], const float (&w2)[3], const float (&old)[3],
+                                          float (&top)[3], float (&g)[3], int r, bool emit, float& acc) {
     float bot[3], top_next[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -363,7 +370,7 @@ __device__ __forceinline__ void kp07u_row(const UnsplitArgs& a, const UStrip<2>&
         x_fluxes_kp07<S, M>(w0, a, F);
 #pragma unroll
         for (int k = 0; k < 3; ++k) out[k] = u_update<M>(w0[k], from_left(F[k]), F[k], g[k], G[k], a);
-        u_store<2>(a, st, st.row0 + r - 2, out);
+        u_store<2, Stage2>(a, st, st.row0 + r - 2, out, old);
         u_cfl<S, M>(out, a, st.lane_out, acc);  // before the RK2 average, as writeCfl reads the tile
     }
 #pragma unroll
@@ -411,25 +418,40 @@ __device__ __forceinline__ float march_gc1(const UnsplitArgs& a, const UStrip<1>
     return acc;
 }
 
-template <class S, class M>
+// Stage2: the output buffer's row of step m (padded row row0 + m - 2) is read
+// with input row m, into slot m % N of its own ring, PF steps before it is
+// averaged and stored.  Only steps m >= 4 emit, so only their rows are read
+// (the rows of steps 0 and 1 would lie above the buffer for the first segment).
+template <class S, class M, bool Stage2>
 __device__ __forceinline__ float march_kp07u(const UnsplitArgs& a, const UStrip<2>& st) {
     constexpr int PF = kRowPrefetch;
     constexpr int N = PF + 3;
+    static_assert(3 + PF >= 4, "rows loaded in the steps belong to emitting steps");
     float acc = M::exact ? __builtin_inff() : 0.0f;
-    float ring[N][3], P[3], G[3];
+    float ring[N][3], old[N][3], P[3], G[3];
     const int last = st.r_end - 1;  // r_end >= 5
+    auto load = [&](int m, int slot) {  // m >= 4
+        u_load<2>(a, st, min(m, last), ring[slot]);
+        if constexpr (Stage2) u_load_old<2>(a, st, st.row0 + min(m, last) - 2, old[slot]);
+    };
 #pragma unroll
-    for (int m = 0; m < N; ++m) u_load<2>(a, st, min(m, last), ring[m]);
+    for (int m = 0; m < N; ++m) {
+        if (m >= 4)
+            load(m, m);
+        else
+            u_load<2>(a, st, min(m, last), ring[m]);
+    }
 #pragma unroll
     for (int k = 0; k < 3; ++k) {  // top face of row 1
         P[k] = ring[1][k] + half_slope<M>(ring[0][k], ring[1][k], ring[2][k], a);
         G[k] = 0.0f;
     }
-    u_load<2>(a, st, min(3 + PF, last), ring[(3 + PF) % N]);
-    kp07u_row<S, M>(a, st, ring[1], ring[2], ring[3 % N], P, G, 3, false, acc);  // flux (1, 2)
+    load(3 + PF, (3 + PF) % N);
+    kp07u_row<S, M, Stage2>(a, st, ring[1], ring[2], ring[3 % N], old[3 % N], P, G, 3, false, acc);  // flux (1, 2)
     auto step = [&](int u, int rr) {
-        u_load<2>(a, st, min(rr + PF, last), ring[(4 + u + PF) % N]);
-        kp07u_row<S, M>(a, st, ring[(2 + u) % N], ring[(3 + u) % N], ring[(4 + u) % N], P, G, rr, true, acc);
+        load(rr + PF, (4 + u + PF) % N);
+        kp07u_row<S, M, Stage2>(a, st, ring[(2 + u) % N], ring[(3 + u) % N], ring[(4 + u) % N], old[(4 + u) % N], P,
+                                G, rr, true, acc);
     };
     int r = 4;
     for (; r + N <= st.r_end; r += N) {
@@ -464,7 +486,7 @@ __global__ void __launch_bounds__(256, 4) unsplit_kernel(UnsplitArgs a) {
     if constexpr (GC == 1)
         acc = march_gc1<S, M>(a, st);
     else
-        acc = march_kp07u<S, M>(a, st);
+        acc = a.stage2 ? march_kp07u<S, M, true>(a, st) : march_kp07u<S, M, false>(a, st);
     float dt_min = acc;
     if constexpr (!M::exact) dt_min = (acc > 0.0f) ? fast::rcp(acc) : __builtin_inff();
     if (a.cfl != nullptr) {
