@@ -198,10 +198,18 @@ __device__ __forceinline__ float wave_min(float v) {
 // Scheme policies.  pred_diff = F(QL) - F(QR) for the half-step predictor;
 // flux_x / flux_y the interface fluxes; cell_dt (exact) or cell_rate = 1/dt
 // (fast) the CFL quantity of an updated cell.
+// kPinRows: the march pins each prefetched row at its first use (use_row);
+// kCarryDiff: the y sweep of the fast policy carries one row's forward
+// difference over as the next row's backward difference.  Measured at 4096^2
+// (SWE) and 8192^2 (Euler): pinning takes the WAF substep from 210 to 197 us and
+// HLL2 about 1 % faster, KP07_dimsplit even; the Euler kernel is 7 % (carry)
+// and 20-25 % (pinned rows) slower with them, so it keeps neither.
 template <class M>
 struct SweBase {
     static constexpr int NV = 3;
     static constexpr bool kEuler = false;
+    static constexpr bool kPinRows = true;
+    static constexpr bool kCarryDiff = true;
     __device__ static void pred_diff(const float QL[3], const float QR[3], const SubstepArgs& a, float D[3]) {
         float FL[3], FR[3];
         swe_F<M>(QL, a.g, FL);
@@ -259,6 +267,8 @@ struct EulerKP07 {
     static constexpr int NV = 4;
     static constexpr bool kSlopes = true;
     static constexpr bool kEuler = true;
+    static constexpr bool kPinRows = false;
+    static constexpr bool kCarryDiff = false;
     // The predictor passes gamma in the pressure slot: EE2D_KP07_dimsplit.cu:57-58, 102-103.
     __device__ static void pred_diff(const float QL[4], const float QR[4], const SubstepArgs& a, float D[4]) {
         float FL[4], FR[4];
@@ -306,14 +316,40 @@ __device__ __forceinline__ void to_y(const float q[NV], float o[NV]) {
     if constexpr (NV == 4) o[3] = q[3];
 }
 
+// Fast-policy reconstruction + half-step predictor of one cell from its
+// backward and forward differences b = c - l, f = r - c:
+//   minmod(theta b, (b + f)/2, theta f) = theta minmod(b, (b + f)/(2 theta), f)
+// for theta > 0, so each neighbour enters once (a cross-lane operand in the x
+// sweep) and the y sweep reuses the forward difference of one row as the
+// backward difference of the next.
+template <class S>
+__device__ __forceinline__ void reconstruct_fast(const float b[S::NV], const float f[S::NV], const float c[S::NV],
+                                                 const SubstepArgs& a, float kk, float qbl[S::NV],
+                                                 float qbr[S::NV]) {
+    constexpr int NV = S::NV;
+    float QL[NV], QR[NV], D[NV], m[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        m[k] = __builtin_amdgcn_fmed3f(__builtin_amdgcn_fmed3f(b[k], f[k], 0.0f), (b[k] + f[k]) * a.inv_2theta, 0.0f);
+        QL[k] = __builtin_fmaf(-a.half_theta, m[k], c[k]);
+        QR[k] = __builtin_fmaf(a.half_theta, m[k], c[k]);
+    }
+    S::pred_diff_fast(QL, QR, c, m, a, D);
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        qbl[k] = QL[k] + kk * D[k];
+        qbr[k] = QR[k] + kk * D[k];
+    }
+}
+
 // Reconstruction + half-step predictor of one cell (computeFluxF, SWE2D_KP07_dimsplit.cu:42-58):
 // slope from (l, c, r); returns the predicted left-face (qbl) and right-face (qbr) states.
 template <class S, class M>
 __device__ __forceinline__ void reconstruct(const float l[S::NV], const float c[S::NV], const float r[S::NV],
                                             const SubstepArgs& a, float kk, float qbl[S::NV], float qbr[S::NV]) {
     constexpr int NV = S::NV;
-    float QL[NV], QR[NV], D[NV];
     if constexpr (M::exact) {
+        float QL[NV], QR[NV], D[NV];
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
             const float s = minmod_slope<M>(l[k], c[k], r[k], a.theta);
@@ -321,11 +357,14 @@ __device__ __forceinline__ void reconstruct(const float l[S::NV], const float c[
             QR[k] = c[k] + 0.5f * s;
         }
         S::pred_diff(QL, QR, a, D);
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            qbl[k] = QL[k] + kk * D[k];
+            qbr[k] = QR[k] + kk * D[k];
+        }
     } else {
-        // minmod(theta b, (b + f)/2, theta f) = theta minmod(b, (b + f)/(2 theta), f)
-        // for theta > 0, with b, f the backward and forward differences; each
-        // neighbour enters once (a cross-lane operand in the x sweep)
-        float m[NV];
+        // as reconstruct_fast with b = c - l, f = r - c
+        float QL[NV], QR[NV], D[NV], m[NV];
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
             const float b = c[k] - l[k], f = r[k] - c[k];
@@ -334,11 +373,11 @@ __device__ __forceinline__ void reconstruct(const float l[S::NV], const float c[
             QR[k] = __builtin_fmaf(a.half_theta, m[k], c[k]);
         }
         S::pred_diff_fast(QL, QR, c, m, a, D);
-    }
 #pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        qbl[k] = QL[k] + kk * D[k];
-        qbr[k] = QR[k] + kk * D[k];
+        for (int k = 0; k < NV; ++k) {
+            qbl[k] = QL[k] + kk * D[k];
+            qbr[k] = QR[k] + kk * D[k];
+        }
     }
 }
 
@@ -415,8 +454,18 @@ struct Strip {
     __amdgpu_buffer_rsrc_t rin[4], rout[4];
 };
 
+// x mirrors flip the x momentum, y mirrors the y momentum
 template <int NV>
-__device__ __forceinline__ void load_row(const SubstepArgs& a, const Strip& st, int r, float v[NV]) {
+__device__ __forceinline__ void apply_signs(const Strip& st, unsigned ysign, float* v) {
+    v[1] = __uint_as_float(__float_as_uint(v[1]) ^ st.xsign);
+    v[2] = __uint_as_float(__float_as_uint(v[2]) ^ ysign);
+}
+
+// Issues the loads of padded row r and returns the sign-bit mask of its y
+// momentum.  Pinned: the raw words are returned and use_row applies the mirror
+// signs at the row's first use; otherwise they are applied here.
+template <int NV, bool Pinned>
+__device__ __forceinline__ unsigned load_row_raw(const SubstepArgs& a, const Strip& st, int r, float v[NV]) {
     // Row offsets are wave-uniform (SGPR soffset); the lane adds its column
     // offset (VGPR voffset): no per-lane address arithmetic.
     unsigned ysign;
@@ -425,10 +474,27 @@ __device__ __forceinline__ void load_row(const SubstepArgs& a, const Strip& st, 
     for (int k = 0; k < NV; ++k) {
         // the b32 buffer builtins move raw 32-bit words
         unsigned x = __builtin_amdgcn_raw_buffer_load_b32(st.rin[k], st.lane_off, row * a.in_pitch[k], 0);
-        // x mirrors flip the x momentum, y mirrors the y momentum
-        if (k == 1) x ^= st.xsign;
-        if (k == 2) x ^= ysign;
+        if constexpr (!Pinned) {
+            if (k == 1) x ^= st.xsign;
+            if (k == 2) x ^= ysign;
+        }
         v[k] = __uint_as_float(x);
+    }
+    return ysign;
+}
+
+// First use of a row loaded by load_row_raw.  Pinned: an empty volatile
+// statement takes the loaded words and hands them on; being volatile it stays
+// behind every load and store issued before it in program order, so the
+// instruction scheduler cannot hoist the row's consumers (and the wait for its
+// data) above the later rows' loads, and each load keeps its distance of
+// kRowPrefetch rows of work.  The mirror signs are applied after it.
+template <int NV, bool Pinned>
+__device__ __forceinline__ void use_row(const Strip& st, unsigned ysign, float (&v)[NV]) {
+    if constexpr (Pinned) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) asm volatile("" : "+v"(v[k]));
+        apply_signs<NV>(st, ysign, v);
     }
 }
 
@@ -472,16 +538,27 @@ static_assert(kRowPrefetch >= 1 && kRowPrefetch <= 3, "row prefetch distance 1..
 template <class S, class M, bool XFirst>
 __device__ __forceinline__ void slope_row(const SubstepArgs& a, const Strip& st, float kky,
                                           const float (&w0)[S::NV], const float (&w1)[S::NV], float (&w2)[S::NV],
-                                          float (&qbt)[S::NV], float (&g)[S::NV], int r, bool emit,
-                                          float& dt_min) {
+                                          float (&qbt)[S::NV], float (&g)[S::NV], float (&dq)[S::NV], int r,
+                                          bool emit, float& dt_min) {
     // w0, w1: rows r-2, r-1 (already x swept when XFirst); w2: raw row r on entry.
+    // dq (fast policy): y-order difference row r-1 minus row r-2 on entry, row r
+    // minus row r-1 on exit.
     constexpr int NV = S::NV;
     if constexpr (XFirst) x_sweep_row<S, M>(w2, a);
     float y0[NV], y1[NV], y2[NV], qbb[NV], qbt_next[NV];
     to_y<NV>(w0, y0);
     to_y<NV>(w1, y1);
     to_y<NV>(w2, y2);
-    reconstruct<S, M>(y0, y1, y2, a, kky, qbb, qbt_next);  // faces of row r-1
+    if constexpr (M::exact || !S::kCarryDiff) {
+        reconstruct<S, M>(y0, y1, y2, a, kky, qbb, qbt_next);  // faces of row r-1
+    } else {
+        float f[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) f[k] = y2[k] - y1[k];
+        reconstruct_fast<S>(dq, f, y1, a, kky, qbb, qbt_next);  // faces of row r-1
+#pragma unroll
+        for (int k = 0; k < NV; ++k) dq[k] = f[k];
+    }
     float G[NV];
     S::flux_y(qbt, qbb, a, G);  // interface (r-2, r-1)
     if (emit) {
@@ -564,11 +641,14 @@ __device__ __forceinline__ float march_slopes(const SubstepArgs& a, const Strip&
     constexpr int N = PF + 3;  // rows r-2, r-1, r and PF rows in flight
     const float kky = M::exact ? a.dt / (2.0f * a.dy) : 0.5f * a.dtdy;
     float dt_min = M::exact ? __builtin_inff() : 0.0f;
-    float ring[N][NV], P[NV], G[NV];
+    float ring[N][NV], P[NV], G[NV], dq[NV];
+    unsigned ys[N];  // y-momentum sign mask of each slot's row
     const int last = st.r_end - 1;  // r_end >= 5
     // prologue: rows 0 .. N-1; faces of row 1; then step r = 3 (flux (1,2), no output)
 #pragma unroll
-    for (int m = 0; m < N; ++m) load_row<NV>(a, st, min(m, last), ring[m]);
+    for (int m = 0; m < N; ++m) ys[m] = load_row_raw<NV, S::kPinRows>(a, st, min(m, last), ring[m]);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) use_row<NV, S::kPinRows>(st, ys[m], ring[m]);
     if constexpr (XFirst) {
         x_sweep_row<S, M>(ring[0], a);
         x_sweep_row<S, M>(ring[1], a);
@@ -580,19 +660,23 @@ __device__ __forceinline__ float march_slopes(const SubstepArgs& a, const Strip&
         to_y<NV>(ring[1], y1);
         to_y<NV>(ring[2], y2);
         reconstruct<S, M>(y0, y1, y2, a, kky, qbb, P);  // faces of row 1
+#pragma unroll
+        for (int k = 0; k < NV; ++k) dq[k] = y2[k] - y1[k];
     }
 #pragma unroll
     for (int k = 0; k < NV; ++k) G[k] = 0.0f;
-    load_row<NV>(a, st, min(3 + PF, last), ring[(3 + PF) % N]);
-    slope_row<S, M, XFirst>(a, st, kky, ring[1], ring[2], ring[3 % N], P, G, 3, false, dt_min);
+    ys[(3 + PF) % N] = load_row_raw<NV, S::kPinRows>(a, st, min(3 + PF, last), ring[(3 + PF) % N]);
+    use_row<NV, S::kPinRows>(st, ys[3 % N], ring[3 % N]);
+    slope_row<S, M, XFirst>(a, st, kky, ring[1], ring[2], ring[3 % N], P, G, dq, 3, false, dt_min);
     // steady state: r = 4, 4 + N, ...; the step of row rr loads row rr + PF into
     // the slot of row rr - 3, whose last use was the previous step.  Whole
     // groups of N rows run without branches; the remainder (< N rows) keeps
     // the same compile-time slots behind wave-uniform tests.
     auto step = [&](int u, int rr) {
-        load_row<NV>(a, st, min(rr + PF, last), ring[(4 + u + PF) % N]);
-        slope_row<S, M, XFirst>(a, st, kky, ring[(2 + u) % N], ring[(3 + u) % N], ring[(4 + u) % N], P, G, rr, true,
-                                dt_min);
+        ys[(4 + u + PF) % N] = load_row_raw<NV, S::kPinRows>(a, st, min(rr + PF, last), ring[(4 + u + PF) % N]);
+        use_row<NV, S::kPinRows>(st, ys[(4 + u) % N], ring[(4 + u) % N]);
+        slope_row<S, M, XFirst>(a, st, kky, ring[(2 + u) % N], ring[(3 + u) % N], ring[(4 + u) % N], P, G, dq, rr,
+                                true, dt_min);
     };
     int r = 4;
     for (; r + N <= st.r_end; r += N) {
@@ -613,11 +697,13 @@ __device__ __forceinline__ float march_waf(const SubstepArgs& a, const Strip& st
     float dt_min = M::exact ? __builtin_inff() : 0.0f;
     float ring[N][3], G[3], hd[2];
     WafCell cell[N];
+    unsigned ys[N];  // y-momentum sign mask of each slot's row
     const int last = st.r_end - 1;
 #pragma unroll
-    for (int m = 0; m < N; ++m) load_row<3>(a, st, min(m, last), ring[m]);
+    for (int m = 0; m < N; ++m) ys[m] = load_row_raw<3, S::kPinRows>(a, st, min(m, last), ring[m]);
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
+        use_row<3, S::kPinRows>(st, ys[m], ring[m]);
         if constexpr (XFirst) x_sweep_row<S, M>(ring[m], a);
         float y[3];
         to_y<3>(ring[m], y);
@@ -627,10 +713,12 @@ __device__ __forceinline__ float march_waf(const SubstepArgs& a, const Strip& st
     hd[1] = waf_hstar<M>(cell[1], cell[2], a.g, a.inv_g);  // interface (1, 2)
 #pragma unroll
     for (int k = 0; k < 3; ++k) G[k] = 0.0f;
+    use_row<3, S::kPinRows>(st, ys[3], ring[3]);
     waf_row<S, M, XFirst>(a, st, ring[0], ring[1], ring[3], cell[0], cell[1], cell[2], cell[3], hd, G, 3, false,
                           dt_min);  // flux (1,2)
     auto step = [&](int u, int rr) {
-        load_row<3>(a, st, min(rr + PF, last), ring[(4 + u + PF) % N]);
+        ys[(4 + u + PF) % N] = load_row_raw<3, S::kPinRows>(a, st, min(rr + PF, last), ring[(4 + u + PF) % N]);
+        use_row<3, S::kPinRows>(st, ys[(4 + u) % N], ring[(4 + u) % N]);
         waf_row<S, M, XFirst>(a, st, ring[(1 + u) % N], ring[(2 + u) % N], ring[(4 + u) % N], cell[(1 + u) % N],
                               cell[(2 + u) % N], cell[(3 + u) % N], cell[(4 + u) % N], hd, G, rr, true, dt_min);
     };
