@@ -24,6 +24,10 @@ class Stream:
     def synchronize(self):
         _lib.call("fvmi_stream_synchronize", self.handle)
 
+    def wait_for_event(self, event):
+        """Work queued after this call waits for ``event``."""
+        _lib.call("fvmi_stream_wait_event", self.handle, event.handle)
+
     def __del__(self):
         if getattr(self, "handle", None) is not None and _lib._lib is not None:
             _lib.lib().fvmi_stream_destroy(self.handle)

@@ -31,6 +31,12 @@ class LaunchConfig(ctypes.Structure):
     _fields_ = [("tile_rows", c_int), ("waves_per_block", c_int), ("math", c_int), ("xcd_remap", c_int)]
 
 
+class RegionCopy(ctypes.Structure):
+    """fvmi_region_copy_t"""
+    _fields_ = [("src", c_void_p), ("dst", c_void_p), ("src_pitch", c_int), ("dst_pitch", c_int),
+                ("width", c_int), ("height", c_int)]
+
+
 class Instance(ctypes.Structure):
     """fvmi_instance_t"""
     _fields_ = [("field", (c_void_p * 4) * 2), ("pitch", c_int), ("nx", c_int), ("ny", c_int), ("bc", c_int),
@@ -86,6 +92,7 @@ SIGNATURES = {
     "fvmi_event_destroy": (c_int, [c_void_p]),
     "fvmi_event_record": (c_int, [c_void_p, c_void_p]),
     "fvmi_event_synchronize": (c_int, [c_void_p]),
+    "fvmi_stream_wait_event": (c_int, [c_void_p, c_void_p]),
     "fvmi_event_elapsed_ms": (c_int, [c_void_p, c_void_p, P(c_float)]),
     "fvmi_malloc": (c_int, [P(c_void_p), c_size_t]),
     "fvmi_malloc_pitch": (c_int, [P(c_void_p), P(c_size_t), c_size_t, c_size_t]),
@@ -96,6 +103,7 @@ SIGNATURES = {
     "fvmi_memcpy2d_async": (c_int, [c_void_p, c_size_t, c_void_p, c_size_t, c_size_t, c_size_t, c_int, c_void_p]),
     "fvmi_fill_f32": (c_int, [c_void_p, c_size_t, c_int, c_int, c_float, c_void_p]),
     "fvmi_copy_f32": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
+    "fvmi_copy_regions_f32": (c_int, [c_void_p, c_int, c_void_p]),
     "fvmi_swe_kp07_dimsplit_substep": (c_int, [c_int, c_int, c_float, c_float, c_float, c_float, c_float, c_int, c_int]
                                        + [c_void_p, c_int] * 6 + [c_void_p, P(LaunchConfig), c_void_p]),
     "fvmi_swe_hll2_substep": (c_int, [c_int, c_int, c_float, c_float, c_float, c_float, c_float, c_int, c_int]

@@ -139,6 +139,24 @@ __global__ void copy_kernel(float4* __restrict__ dst, const float4* __restrict__
         dst[i] = src[i];
 }
 
+// Batched 2-D region copies (SHMEMSimulator's exchange): up to kMaxRegions
+// regions per launch, passed by value; block row y copies region y.
+constexpr int kMaxRegions = 64;
+struct RegionBatch {
+    fvmi_region_copy_t r[kMaxRegions];
+};
+
+__global__ void copy_regions_kernel(RegionBatch b) {
+    const fvmi_region_copy_t R = b.r[blockIdx.y];
+    const long long n = (long long)R.width * R.height;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int row = (int)(i / R.width);
+        const int col = (int)(i - (long long)row * R.width);
+        const float v = reinterpret_cast<const float*>(reinterpret_cast<const char*>(R.src) + (size_t)row * R.src_pitch)[col];
+        reinterpret_cast<float*>(reinterpret_cast<char*>(R.dst) + (size_t)row * R.dst_pitch)[col] = v;
+    }
+}
+
 struct HaloArgs {
     float* f[4];
     int pitch[4];
@@ -428,6 +446,12 @@ int fvmi_event_record(void* event, void* stream) {
     return FVMI_OK;
 }
 
+int fvmi_stream_wait_event(void* stream, void* event) {
+    if (!event) return fail(FVMI_ERR_INVALID, "event is NULL");
+    FVMI_HIP(hipStreamWaitEvent((hipStream_t)stream, (hipEvent_t)event, 0));
+    return FVMI_OK;
+}
+
 int fvmi_event_synchronize(void* event) {
     FVMI_HIP(hipEventSynchronize((hipEvent_t)event));
     return FVMI_OK;
@@ -628,6 +652,35 @@ int fvmi_halo_exchange_periodic(int nx, int ny, int nvars, float* const* fields,
     const int frame = 4 * (nx + 4) + 4 * ny;
     hipLaunchKernelGGL(halo_periodic_kernel, dim3((frame + 255) / 256, nvars), dim3(256), 0, (hipStream_t)stream, a);
     FVMI_HIP(hipGetLastError());
+    return FVMI_OK;
+}
+
+int fvmi_copy_regions_f32(const fvmi_region_copy_t* regions, int count, void* stream) {
+    if (count < 0 || (count > 0 && !regions)) return fail(FVMI_ERR_INVALID, "bad region list");
+    for (int i = 0; i < count; ++i) {
+        const fvmi_region_copy_t& R = regions[i];
+        if (R.width < 0 || R.height < 0) return fail(FVMI_ERR_INVALID, "region %d has a negative size", i);
+        if (R.width == 0 || R.height == 0) continue;
+        if (!R.src || !R.dst) return fail(FVMI_ERR_INVALID, "region %d has a null pointer", i);
+        if (R.src_pitch < R.width * (int)sizeof(float) || R.dst_pitch < R.width * (int)sizeof(float))
+            return fail(FVMI_ERR_INVALID, "region %d: pitch below the row width", i);
+    }
+    for (int first = 0; first < count; first += kMaxRegions) {
+        const int n = count - first < kMaxRegions ? count - first : kMaxRegions;
+        RegionBatch b{};
+        long long most = 0;
+        for (int i = 0; i < n; ++i) {
+            b.r[i] = regions[first + i];
+            if (b.r[i].width <= 0 || b.r[i].height <= 0) b.r[i].width = b.r[i].height = 0;
+            const long long e = (long long)b.r[i].width * b.r[i].height;
+            most = e > most ? e : most;
+        }
+        if (most == 0) continue;
+        long long bx = (most + 255) / 256;
+        if (bx > 64) bx = 64;
+        hipLaunchKernelGGL(copy_regions_kernel, dim3((unsigned)bx, n), dim3(256), 0, (hipStream_t)stream, b);
+        FVMI_HIP(hipGetLastError());
+    }
     return FVMI_OK;
 }
 

@@ -82,6 +82,8 @@ int fvmi_event_create(void** event);
 int fvmi_event_destroy(void* event);
 int fvmi_event_record(void* event, void* stream);
 int fvmi_event_synchronize(void* event);
+/* Work queued on `stream` after this call waits for `event` (cuda.Stream.wait_for_event). */
+int fvmi_stream_wait_event(void* stream, void* event);
 int fvmi_event_elapsed_ms(void* start, void* end, float* ms);
 
 /* ---- memory: gpuarray.zeros / free (Common.py:510, 535-538), Memcpy2D (:543-616) */
@@ -187,6 +189,24 @@ int fvmi_euler_kp07_dimsplit_substep(int nx, int ny, float dx, float dy, float d
  * the `nvars` fields copy padded columns [nx, nx+2) -> [0, 2) and
  * [2, 4) -> [nx+2, nx+4) over all ny+4 rows, on the device. */
 int fvmi_halo_copy_periodic_x(int nx, int ny, int nvars, float* const* fields, const int* pitches, void* stream);
+
+/* One rectangular block of float32 cells: `height` rows of `width` values,
+ * row pitches in bytes. */
+typedef struct fvmi_region_copy {
+    const float* src;
+    float* dst;
+    int src_pitch;
+    int dst_pitch;
+    int width;
+    int height;
+} fvmi_region_copy_t;
+
+/* SHMEMSimulator's exchange on one device (SHMEMSimulator.py:200-263, where
+ * each region goes through host memory with CudaArray2D.download/upload,
+ * Common.py:543-616): copies `count` regions device to device in launches of
+ * at most 64 regions, in list order between launches.  Regions of one call
+ * must not overlap a region another of them writes. */
+int fvmi_copy_regions_f32(const fvmi_region_copy_t* regions, int count, void* stream);
 
 /* The same exchange for a rank that is its own neighbour in x and/or y
  * (MPIGrid with one rank and periodic edges, MPISimulator.py:247-309): with
