@@ -1,9 +1,10 @@
-// KP07_dimsplit and HLL2 substep kernels (single grid and ensemble) in fast
-// arithmetic, built with the ILP-maximising instruction scheduler: 4 % faster
-// on the 4096^2 KP07 step and 6 % on the KP07 ensemble; WAF and the Euler
-// kernel gain nothing from it (the Euler kernel loses 2 %) and stay in
-// fv_kernels_fast.hip with the default scheduler.
+// Shallow-water kernels in fast arithmetic (KP07_dimsplit, HLL2 and WAF, single
+// grid and ensemble; LxF, HLL, FORCE and the unsplit KP07), built with the
+// ILP-maximising instruction scheduler.  Measured at 4096^2 against the default
+// scheduler: KP07_dimsplit 4 % faster, its ensemble 6 %; WAF and LxF 2 % with
+// the pinned row march.  The Euler kernel stays in fv_kernels_fast.hip.
 #include "fv_launch.h"
+#include "fv_unsplit.h"
 
 namespace fvmi {
 hipError_t launch_substep_fast_ilp(int scheme, const SubstepArgs& a, int grid, hipStream_t stream) {
@@ -12,6 +13,8 @@ hipError_t launch_substep_fast_ilp(int scheme, const SubstepArgs& a, int grid, h
             return launch_substep_scheme<SweKP07<MathFast>, MathFast>(a, grid, stream);
         case SCHEME_HLL2:
             return launch_substep_scheme<SweHLL2<MathFast>, MathFast>(a, grid, stream);
+        case SCHEME_WAF:
+            return launch_substep_scheme<SweWAF<MathFast>, MathFast>(a, grid, stream);
         default:
             return hipErrorInvalidValue;
     }
@@ -22,8 +25,13 @@ hipError_t launch_ensemble_fast_ilp(int scheme, const EnsembleArgs& e, int grid,
             return launch_ensemble_scheme<SweKP07<MathFast>, MathFast>(e, grid, stream);
         case SCHEME_HLL2:
             return launch_ensemble_scheme<SweHLL2<MathFast>, MathFast>(e, grid, stream);
+        case SCHEME_WAF:
+            return launch_ensemble_scheme<SweWAF<MathFast>, MathFast>(e, grid, stream);
         default:
             return hipErrorInvalidValue;
     }
+}
+hipError_t launch_unsplit_fast(int scheme, const UnsplitArgs& a, int grid, hipStream_t stream) {
+    return launch_unsplit<MathFast>(scheme, a, grid, stream);
 }
 }  // namespace fvmi

@@ -191,6 +191,16 @@ __device__ __forceinline__ float wave_min(float v) {
     return v;
 }
 
+// Rows loaded ahead of the row being processed.  Each wave keeps a ring of
+// row slots in registers: the rows its y stencil still needs plus this many
+// rows in flight, so a load is issued kRowPrefetch rows of work before its
+// first use.
+#ifndef FVMI_ROW_PREFETCH
+#define FVMI_ROW_PREFETCH 2
+#endif
+constexpr int kRowPrefetch = FVMI_ROW_PREFETCH;
+static_assert(kRowPrefetch >= 1 && kRowPrefetch <= 3, "row prefetch distance 1..3");
+
 // ---------------------------------------------------------------------------
 // Scheme policies
 // ---------------------------------------------------------------------------
@@ -200,16 +210,19 @@ __device__ __forceinline__ float wave_min(float v) {
 // (fast) the CFL quantity of an updated cell.
 // kPinRows: the march pins each prefetched row at its first use (use_row);
 // kCarryDiff: the y sweep of the fast policy carries one row's forward
-// difference over as the next row's backward difference.  Measured at 4096^2
-// (SWE) and 8192^2 (Euler): pinning takes the WAF substep from 210 to 197 us and
-// HLL2 about 1 % faster, KP07_dimsplit even; the Euler kernel is 7 % (carry)
-// and 20-25 % (pinned rows) slower with them, so it keeps neither.
+// difference over as the next row's backward difference; kPrefetch: rows
+// loaded ahead (kRowPrefetch).  Measured at 4096^2 (SWE) and 8192^2 (Euler):
+// pinning takes the WAF substep from 210 to 197 us and HLL2 about 1 % faster,
+// KP07_dimsplit even; the Euler kernel is 7 % (carry) and 20-25 % (pinned rows)
+// slower with them, so it keeps neither, and 1.5 % faster with one row ahead
+// instead of two.
 template <class M>
 struct SweBase {
     static constexpr int NV = 3;
     static constexpr bool kEuler = false;
     static constexpr bool kPinRows = true;
     static constexpr bool kCarryDiff = true;
+    static constexpr int kPrefetch = kRowPrefetch;
     __device__ static void pred_diff(const float QL[3], const float QR[3], const SubstepArgs& a, float D[3]) {
         float FL[3], FR[3];
         swe_F<M>(QL, a.g, FL);
@@ -269,6 +282,7 @@ struct EulerKP07 {
     static constexpr bool kEuler = true;
     static constexpr bool kPinRows = false;
     static constexpr bool kCarryDiff = false;
+    static constexpr int kPrefetch = 1;
     // The predictor passes gamma in the pressure slot: EE2D_KP07_dimsplit.cu:57-58, 102-103.
     __device__ static void pred_diff(const float QL[4], const float QR[4], const SubstepArgs& a, float D[4]) {
         float FL[4], FR[4];
@@ -522,16 +536,6 @@ __device__ __forceinline__ void accumulate_cfl(const float (&q)[S::NV], const Su
     }
 }
 
-// Rows loaded ahead of the row being processed.  Each wave keeps a ring of
-// row slots in registers: the rows its y stencil still needs plus this many
-// rows in flight, so a load is issued kRowPrefetch rows of work before its
-// first use.
-#ifndef FVMI_ROW_PREFETCH
-#define FVMI_ROW_PREFETCH 2
-#endif
-constexpr int kRowPrefetch = FVMI_ROW_PREFETCH;
-static_assert(kRowPrefetch >= 1 && kRowPrefetch <= 3, "row prefetch distance 1..3");
-
 // One row step of the y sweep for slope schemes.  Loop-carried state: qbt, the
 // predicted top face of row r-2, and g, the flux through its bottom interface;
 // both are replaced by the values of row r-1 (in place: reads precede writes).
@@ -637,7 +641,7 @@ __device__ __forceinline__ void copyv(float (&d)[NV], const float (&s)[NV]) {
 template <class S, class M, bool XFirst>
 __device__ __forceinline__ float march_slopes(const SubstepArgs& a, const Strip& st) {
     constexpr int NV = S::NV;
-    constexpr int PF = kRowPrefetch;
+    constexpr int PF = S::kPrefetch;
     constexpr int N = PF + 3;  // rows r-2, r-1, r and PF rows in flight
     const float kky = M::exact ? a.dt / (2.0f * a.dy) : 0.5f * a.dtdy;
     float dt_min = M::exact ? __builtin_inff() : 0.0f;
@@ -692,7 +696,7 @@ __device__ __forceinline__ float march_slopes(const SubstepArgs& a, const Strip&
 // Row march of WAF (4-row window): row m and its WafCell in ring slot m % N.
 template <class S, class M, bool XFirst>
 __device__ __forceinline__ float march_waf(const SubstepArgs& a, const Strip& st) {
-    constexpr int PF = kRowPrefetch;
+    constexpr int PF = S::kPrefetch;
     constexpr int N = PF + 4;  // rows r-3 .. r and PF rows in flight
     float dt_min = M::exact ? __builtin_inff() : 0.0f;
     float ring[N][3], G[3], hd[2];
