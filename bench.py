@@ -218,10 +218,10 @@ def _ensemble_leg(ctx, count):
     insts = ensemble_instances(count, seed=0)
     specs = [{"fields": [d["h"], d["hu"], d["hv"]], "nx": d["nx"], "ny": d["ny"], "dx": 1.0, "dy": 1.0,
               "g": 9.81, "t_end": d["t_end"], "bc": d["bc"]} for d in insts]
-    # schedule from the autotuner, timed on every 8th instance (untimed here);
+    # schedule from the autotuner, timed on every 4th instance (untimed here);
     # fixed segment lengths only: the balanced plan's length depends on the
     # instance count, so a sample does not represent it
-    sched = Autotuner.ensemble_schedule(ctx, "kp07_dimsplit", specs[::8], seg_rows=(16, 24, 32, 48))
+    sched = Autotuner.ensemble_schedule(ctx, "kp07_dimsplit", specs[::4], seg_rows=(16, 24, 32, 48), blocks=4)
     print("# ensemble schedule: %s" % json.dumps(sched), file=sys.stderr, flush=True)
     ens = Ensemble(ctx, "kp07_dimsplit", specs, math="fast", steps_per_graph=8, seg_rows=sched["seg_rows"],
                    waves_per_block=sched["waves_per_block"])

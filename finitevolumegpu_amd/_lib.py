@@ -104,6 +104,8 @@ SIGNATURES = {
     "fvmi_fill_f32": (c_int, [c_void_p, c_size_t, c_int, c_int, c_float, c_void_p]),
     "fvmi_copy_f32": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
     "fvmi_copy_regions_f32": (c_int, [c_void_p, c_int, c_void_p]),
+    "fvmi_substep_plan": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, c_int, P(LaunchConfig), c_int,
+                                  P(c_int), c_int, P(c_int)]),
     "fvmi_swe_kp07_dimsplit_substep": (c_int, [c_int, c_int, c_float, c_float, c_float, c_float, c_float, c_int, c_int]
                                        + [c_void_p, c_int] * 6 + [c_void_p, P(LaunchConfig), c_void_p]),
     "fvmi_swe_hll2_substep": (c_int, [c_int, c_int, c_float, c_float, c_float, c_float, c_float, c_int, c_int]

@@ -11,19 +11,45 @@ namespace fvmi {
 enum SchemeId : int { SCHEME_KP07 = 0, SCHEME_HLL2 = 1, SCHEME_WAF = 2, SCHEME_EULER = 3 };
 
 // Fills the decomposition fields of `a` (sub-area, launch shape, device size)
-// and returns the grid size to launch.  seg_rows == 0 picks the balanced
-// segment length: about `waves_per_simd` waves on every SIMD of the device,
-// but no segment shorter than `min_seg` rows (the 4-row prologue of a segment
-// is then at most 1/6 of its work).
-inline int plan_substep(SubstepArgs& a, int cu_count, int waves_per_simd = 4, int min_seg = 24) {
+// and returns the grid size to launch.  seg_rows == 0 picks the shape:
+//   * the round-weighted plan (round_segment) when `round_ok`, the workgroups
+//     have 4 waves, the XCD remap is on and every strip holds row groups long
+//     enough for their shortest part to have `min_part` rows;
+//   * otherwise the balanced segment length: at most `waves_per_simd` waves on
+//     every SIMD of the device, but no segment shorter than `min_seg` rows (the
+//     4-row prologue of a segment is then at most 1/6 of its work).
+inline int plan_substep(SubstepArgs& a, int cu_count, bool round_ok = false, int waves_per_simd = 4,
+                        int min_seg = 24, int min_part = 16) {
     const int wx = a.x1 - a.x0;
     a.rows_per_strip = a.y1 - a.y0;
     a.strips_x = (wx + kStripCols - 1) / kStripCols;
+    a.round_plan = 0;
+    a.xcds = 8;
+    a.cus_per_xcd = cu_count / 8;
+    a.gps = a.grows = 0;
+    if (round_ok && a.seg_rows <= 0 && a.waves_per_block == kRounds && a.xcd_remap && cu_count % 8 == 0 &&
+        cu_count > 0) {
+        const int gps = (cu_count * a.waves_per_block) / a.strips_x;
+        const int grows = gps > 0 ? (a.rows_per_strip + gps - 1) / gps : 0;
+        // the shortest part of the shortest (last) group
+        const int last = gps > 0 ? a.rows_per_strip - (gps - 1) * grows : 0;
+        if (gps > 0 && (last * kRoundWeights[kRounds - 1]) / kRoundWeightSum >= min_part) {
+            a.round_plan = 1;
+            a.gps = gps;
+            a.grows = grows;
+            a.seg_rows = grows / kRounds;  // nominal, for reporting
+            a.segs_per_strip = gps * kRounds;
+            a.num_waves = a.strips_x * gps * kRounds;
+            a.num_blocks = kRounds * cu_count;
+            return a.num_blocks;
+        }
+    }
     if (a.seg_rows <= 0) {
-        // about waves_per_simd waves on every SIMD: segments per strip rounded
-        // to the nearest count, then equal segment heights of at least min_seg
+        // at most waves_per_simd waves on every SIMD (one more would run after
+        // all the others): segments per strip rounded down, then equal
+        // segment heights of at least min_seg
         const int target = cu_count * 4 * waves_per_simd;
-        int k = (target + a.strips_x / 2) / a.strips_x;
+        int k = target / a.strips_x;
         if (k < 1) k = 1;
         int seg = (a.rows_per_strip + k - 1) / k;
         a.seg_rows = seg < min_seg ? min_seg : seg;

@@ -62,6 +62,13 @@ struct SubstepArgs {
     int waves_per_block;
     int num_blocks;  // logical blocks = ceil(num_waves / waves_per_block)
     int xcd_remap;
+    // Round-weighted plan (round_plan = 1, see round_segment): the grid is
+    // kRounds blocks per CU; strips are cut into gps row groups of grows rows
+    // (the last one shorter), and part k of a group goes to a wave of dispatch
+    // round k.
+    int round_plan;
+    int xcds, cus_per_xcd;
+    int gps, grows;
     // derived constants for the fast policy (set by the host launcher)
     float hg, sqrt_g, inv_g, inv_dx, inv_dy, dtdx, dtdy;
     float half_theta;  // theta / 2
@@ -762,25 +769,85 @@ __device__ __forceinline__ void setup_strip(const SubstepArgs& a, Strip& st, int
 #define FVMI_SUBSTEP_MIN_WAVES 4
 #endif
 
+// Round-weighted work split.  Workgroups are dispatched in index order, one
+// per CU per round (workgroup p lands on XCD p % xcds, in round
+// (p / xcds) / cus_per_xcd), and a SIMD's instruction arbiter favours its
+// oldest wave, so with equal segments a SIMD's waves finish one after another
+// and the last runs alone.  Here the waves of earlier rounds take longer parts
+// of each row group (weights kRoundWeights), and a CU's waves take
+// consecutive groups of one XCD's contiguous range of strips.
+constexpr int kRounds = 4;
+constexpr int kRoundWeights[kRounds] = {13, 11, 9, 7};
+constexpr int kRoundWeightSum = 13 + 11 + 9 + 7;
+
+// rows [lo, hi) of part k of a group of n rows
+__host__ __device__ inline void round_part(int n, int k, int& lo, int& hi) {
+    int c[kRounds + 1];
+    c[0] = 0;
+    for (int j = 0; j < kRounds - 1; ++j) c[j + 1] = c[j] + (n * kRoundWeights[j]) / kRoundWeightSum;
+    c[kRounds] = n;
+    lo = c[k];
+    hi = c[k + 1];
+}
+
+// the segment of physical wave `wave` of workgroup `block`; false for an idle slot
+__host__ __device__ inline bool round_segment(const SubstepArgs& a, int block, int wave, int& strip_x, int& row_in,
+                                              int& count) {
+    const int x = block % a.xcds;
+    const int q = block / a.xcds;
+    const int k = q / a.cus_per_xcd;
+    const int c = q - k * a.cus_per_xcd;
+    if (k >= kRounds) return false;
+    const int G = (x * a.cus_per_xcd + c) * a.waves_per_block + wave;
+    if (G >= a.strips_x * a.gps) return false;
+    strip_x = G / a.gps;
+    const int g0 = (G - strip_x * a.gps) * a.grows;
+    const int n = min(a.grows, a.rows_per_strip - g0);
+    if (n <= 0) return false;
+    int lo, hi;
+    round_part(n, k, lo, hi);
+    row_in = g0 + lo;
+    count = hi - lo;
+    return count > 0;
+}
+
+// The segment of physical wave `wave` of workgroup `block` (of `grid`) in the
+// equal-segment plan.  Hardware dispatch places workgroup i on XCD i % 8; with
+// xcd_remap each XCD owns a contiguous range of logical blocks (neighbouring
+// strips share L2).
+__host__ __device__ inline bool uniform_segment(const SubstepArgs& a, int block, int grid, int wave, int& strip_x,
+                                                int& row_in, int& count) {
+    int lb = block;
+    if (a.xcd_remap) {
+        const int per = grid >> 3;  // grid is a multiple of 8 when remapping
+        lb = (block & 7) * per + (block >> 3);
+    }
+    const int gw = lb * a.waves_per_block + wave;
+    if (lb >= a.num_blocks || gw >= a.num_waves) return false;
+    strip_x = gw / a.segs_per_strip;
+    row_in = (gw - strip_x * a.segs_per_strip) * a.seg_rows;
+    count = min(a.seg_rows, a.rows_per_strip - row_in);
+    return true;
+}
+
+// the segment of a physical wave in either plan
+__host__ __device__ inline bool wave_segment(const SubstepArgs& a, int block, int grid, int wave, int& strip_x,
+                                             int& row_in, int& count) {
+    return a.round_plan ? round_segment(a, block, wave, strip_x, row_in, count)
+                        : uniform_segment(a, block, grid, wave, strip_x, row_in, count);
+}
+
 template <class S, class M>
 __global__ void __launch_bounds__(256, FVMI_SUBSTEP_MIN_WAVES) substep_kernel(SubstepArgs a) {
     const int lane = threadIdx.x & (kLanes - 1);
     // wave index made scalar so that every row address is computed in SGPRs
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (a.cfl != nullptr && blockIdx.x == 0 && threadIdx.x == 0) a.cfl[1 - (a.step & 1)] = __builtin_inff();
-    // Hardware dispatch places workgroup i on XCD i % 8; remap so that each XCD
-    // owns a contiguous range of logical blocks (neighbouring strips share L2).
-    int lb = blockIdx.x;
-    if (a.xcd_remap) {
-        const int per = gridDim.x >> 3;  // gridDim.x is a multiple of 8 when remapping
-        lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    }
-    const int gw = __builtin_amdgcn_readfirstlane(lb * a.waves_per_block + wave);
-    if (lb >= a.num_blocks || gw >= a.num_waves) return;
-
-    const int strip_x = __builtin_amdgcn_readfirstlane(gw / a.segs_per_strip);
-    const int row_in = __builtin_amdgcn_readfirstlane((gw - strip_x * a.segs_per_strip) * a.seg_rows);
-    const int count = min(a.seg_rows, a.rows_per_strip - row_in);
+    int strip_x, row_in, count;
+    if (!wave_segment(a, blockIdx.x, gridDim.x, wave, strip_x, row_in, count)) return;
+    strip_x = __builtin_amdgcn_readfirstlane(strip_x);
+    row_in = __builtin_amdgcn_readfirstlane(row_in);
+    count = __builtin_amdgcn_readfirstlane(count);
     Strip st;
     setup_strip<S, M>(a, st, lane, strip_x, row_in, count);
     float acc;

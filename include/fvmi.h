@@ -263,6 +263,18 @@ int fvmi_ensemble_prepare(fvmi_instance_t* table, int n, void* stream);
  * evenly over the XCDs. */
 int fvmi_ensemble_substep(int scheme, fvmi_instance_t* table, const int* segs, const int* num_segs_dev, int waves,
                           int step, const fvmi_launch_t* launch, void* stream);
+/* The work split of one substep launch of the four dimensionally split entry
+ * points, computed on the host with no device call: the planner's choice for
+ * scheme (0 KP07_dimsplit, 1 HLL2, 2 WAF, 3 Euler), sub-area [x0,x1)x[y0,y1)
+ * of an nx x ny domain (x1 = y1 = 0: the whole domain), the launch shape and a
+ * device of `cu_count` CUs.  For every wave of the grid, 3 ints in `segs`
+ * (capacity `max_waves` waves): strip, first row and row count relative to
+ * the sub-area, or (-1, 0, 0) for a wave with nothing to do; *n_waves receives
+ * the grid's wave count.  No counterpart in the reference (its tile grid is
+ * ceil(n / block) in each direction); for tests and tools that check that the
+ * segments cover the sub-area once. */
+int fvmi_substep_plan(int scheme, int nx, int ny, int x0, int y0, int x1, int y1, const fvmi_launch_t* launch,
+                      int cu_count, int* segs, int max_waves, int* n_waves);
 /* t += dt_step, nt += 1 for every instance that stepped. */
 int fvmi_ensemble_advance(fvmi_instance_t* table, int n, void* stream);
 /* Writes to *count_dev the number of instances that still have to step. */
