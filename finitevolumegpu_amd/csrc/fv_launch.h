@@ -27,22 +27,17 @@ inline int plan_substep(SubstepArgs& a, int cu_count, bool round_ok = false, int
     a.xcds = 8;
     a.cus_per_xcd = cu_count / 8;
     a.gps = a.grows = 0;
-    if (round_ok && a.seg_rows <= 0 && a.waves_per_block == kRounds && a.xcd_remap && cu_count % 8 == 0 &&
-        cu_count > 0) {
-        const int gps = (cu_count * a.waves_per_block) / a.strips_x;
-        const int grows = gps > 0 ? (a.rows_per_strip + gps - 1) / gps : 0;
-        // the shortest part of the shortest (last) group
-        const int last = gps > 0 ? a.rows_per_strip - (gps - 1) * grows : 0;
-        if (gps > 0 && (last * kRoundWeights[kRounds - 1]) / kRoundWeightSum >= min_part) {
-            a.round_plan = 1;
-            a.gps = gps;
-            a.grows = grows;
-            a.seg_rows = grows / kRounds;  // nominal, for reporting
-            a.segs_per_strip = gps * kRounds;
-            a.num_waves = a.strips_x * gps * kRounds;
-            a.num_blocks = kRounds * cu_count;
-            return a.num_blocks;
-        }
+    int gps = 0, grows = 0;
+    if (round_ok && a.seg_rows <= 0 && a.xcd_remap &&
+        round_split_fits(a.strips_x, a.rows_per_strip, a.waves_per_block, cu_count, min_part, gps, grows)) {
+        a.round_plan = 1;
+        a.gps = gps;
+        a.grows = grows;
+        a.seg_rows = grows / kRounds;  // nominal, for reporting
+        a.segs_per_strip = gps * kRounds;
+        a.num_waves = a.strips_x * gps * kRounds;
+        a.num_blocks = kRounds * cu_count;
+        return a.num_blocks;
     }
     if (a.seg_rows <= 0) {
         // at most waves_per_simd waves on every SIMD (one more would run after

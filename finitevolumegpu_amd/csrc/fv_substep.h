@@ -790,25 +790,46 @@ __host__ __device__ inline void round_part(int n, int k, int& lo, int& hi) {
     hi = c[k + 1];
 }
 
-// the segment of physical wave `wave` of workgroup `block`; false for an idle slot
-__host__ __device__ inline bool round_segment(const SubstepArgs& a, int block, int wave, int& strip_x, int& row_in,
-                                              int& count) {
-    const int x = block % a.xcds;
-    const int q = block / a.xcds;
-    const int k = q / a.cus_per_xcd;
-    const int c = q - k * a.cus_per_xcd;
+// The segment of physical wave `wave` of workgroup `block` in the
+// round-weighted split of `strips` strips of `rows` rows into `gps` groups of
+// `grows` rows; false for an idle slot.
+__host__ __device__ inline bool round_segment_of(int block, int wave, int xcds, int cus_per_xcd, int waves_per_block,
+                                                 int strips, int rows, int gps, int grows, int& strip_x, int& row_in,
+                                                 int& count) {
+    const int x = block % xcds;
+    const int q = block / xcds;
+    const int k = q / cus_per_xcd;
+    const int c = q - k * cus_per_xcd;
     if (k >= kRounds) return false;
-    const int G = (x * a.cus_per_xcd + c) * a.waves_per_block + wave;
-    if (G >= a.strips_x * a.gps) return false;
-    strip_x = G / a.gps;
-    const int g0 = (G - strip_x * a.gps) * a.grows;
-    const int n = min(a.grows, a.rows_per_strip - g0);
+    const int G = (x * cus_per_xcd + c) * waves_per_block + wave;
+    if (G >= strips * gps) return false;
+    strip_x = G / gps;
+    const int g0 = (G - strip_x * gps) * grows;
+    const int n = min(grows, rows - g0);
     if (n <= 0) return false;
     int lo, hi;
     round_part(n, k, lo, hi);
     row_in = g0 + lo;
     count = hi - lo;
     return count > 0;
+}
+
+// Whether the round-weighted split fits: row groups (gps per strip, grows rows)
+// whose shortest part has at least `min_part` rows.
+__host__ __device__ inline bool round_split_fits(int strips, int rows, int waves_per_block, int cu_count, int min_part,
+                                                 int& gps, int& grows) {
+    if (waves_per_block != kRounds || cu_count <= 0 || cu_count % 8 != 0 || strips <= 0) return false;
+    gps = (cu_count * waves_per_block) / strips;
+    if (gps <= 0) return false;
+    grows = (rows + gps - 1) / gps;
+    const int last = rows - (gps - 1) * grows;  // the shortest (last) group
+    return (last * kRoundWeights[kRounds - 1]) / kRoundWeightSum >= min_part;
+}
+
+__host__ __device__ inline bool round_segment(const SubstepArgs& a, int block, int wave, int& strip_x, int& row_in,
+                                              int& count) {
+    return round_segment_of(block, wave, a.xcds, a.cus_per_xcd, a.waves_per_block, a.strips_x, a.rows_per_strip,
+                            a.gps, a.grows, strip_x, row_in, count);
 }
 
 // The segment of physical wave `wave` of workgroup `block` (of `grid`) in the

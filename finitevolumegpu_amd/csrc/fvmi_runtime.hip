@@ -317,7 +317,7 @@ int run_unsplit(int scheme, UnsplitArgs& a, int gc, const fvmi_launch_t* launch,
         a.lw_kx = a.dt / (2.0f * a.dx);
         a.lw_ky = a.dt / (2.0f * a.dy);
     }
-    const int grid = plan_unsplit(a, gc, current_cu_count());
+    const int grid = plan_unsplit(a, gc, current_cu_count(), unsplit_round_ok(scheme));
     hipError_t e = (math == FVMI_MATH_EXACT) ? launch_unsplit_exact(scheme, a, grid, (hipStream_t)stream)
                                              : launch_unsplit_fast(scheme, a, grid, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "unsplit launch");
@@ -699,9 +699,37 @@ int fvmi_ensemble_prepare(fvmi_instance_t* table, int n, void* stream) {
 
 int fvmi_substep_plan(int scheme, int nx, int ny, int x0, int y0, int x1, int y1, const fvmi_launch_t* launch,
                       int cu_count, int* segs, int max_waves, int* n_waves) {
-    if (scheme < SCHEME_KP07 || scheme > SCHEME_EULER) return fail(FVMI_ERR_INVALID, "unknown scheme %d", scheme);
+    if (scheme < SCHEME_KP07 || scheme > SCHEME_EULER + 4) return fail(FVMI_ERR_INVALID, "unknown scheme %d", scheme);
     if (nx < 2 || ny < 2 || cu_count < 1 || !n_waves || max_waves < 0 || (max_waves > 0 && !segs))
         return fail(FVMI_ERR_INVALID, "bad plan arguments");
+    if (scheme > SCHEME_EULER) {
+        // LxF, HLL, FORCE (one ghost cell) and the unsplit KP07 (two): the whole domain
+        if (x0 != 0 || y0 != 0 || (x1 != 0 && x1 != nx) || (y1 != 0 && y1 != ny))
+            return fail(FVMI_ERR_INVALID, "the unsplit schemes update the whole domain");
+        UnsplitArgs u{};
+        u.nx = nx, u.ny = ny;
+        SubstepArgs shape{};
+        int math = 0;
+        if (int rc = fill_launch(launch, shape, &math)) return rc;
+        u.seg_rows = shape.seg_rows;
+        u.waves_per_block = shape.waves_per_block;
+        u.xcd_remap = shape.xcd_remap;
+        const int grid = plan_unsplit(u, scheme == SCHEME_EULER + 4 ? 2 : 1, cu_count,
+                                      unsplit_round_ok(scheme - (SCHEME_EULER + 1)));
+        const long long total = (long long)grid * u.waves_per_block;
+        if (total > (1LL << 30)) return fail(FVMI_ERR_INVALID, "plan too large");
+        *n_waves = (int)total;
+        if (total > max_waves) return fail(FVMI_ERR_INVALID, "segs holds %d waves, the plan has %lld", max_waves, total);
+        for (int b = 0; b < grid; ++b) {
+            for (int w = 0; w < u.waves_per_block; ++w) {
+                int strip = -1, row = 0, count = 0;
+                if (!unsplit_segment(u, b, grid, w, strip, row, count)) strip = -1, row = 0, count = 0;
+                int* o = segs + 3 * ((long long)b * u.waves_per_block + w);
+                o[0] = strip, o[1] = row, o[2] = count;
+            }
+        }
+        return FVMI_OK;
+    }
     SubstepArgs a{};
     a.nx = nx, a.ny = ny, a.x0 = x0, a.y0 = y0;
     a.x1 = x1 == 0 ? nx : x1;

@@ -264,10 +264,12 @@ int fvmi_ensemble_prepare(fvmi_instance_t* table, int n, void* stream);
 int fvmi_ensemble_substep(int scheme, fvmi_instance_t* table, const int* segs, const int* num_segs_dev, int waves,
                           int step, const fvmi_launch_t* launch, void* stream);
 /* The work split of one substep launch of the four dimensionally split entry
- * points, computed on the host with no device call: the planner's choice for
- * scheme (0 KP07_dimsplit, 1 HLL2, 2 WAF, 3 Euler), sub-area [x0,x1)x[y0,y1)
- * of an nx x ny domain (x1 = y1 = 0: the whole domain), the launch shape and a
- * device of `cu_count` CUs.  For every wave of the grid, 3 ints in `segs`
+ * points and of the unsplit ones, computed on the host with no device call:
+ * the planner's choice for scheme (0 KP07_dimsplit, 1 HLL2, 2 WAF, 3 Euler;
+ * 4, 5, 6 LxF, HLL, FORCE, whose strips are 62 columns wide; 7 the unsplit
+ * KP07, 60), sub-area [x0,x1)x[y0,y1) of an nx x ny domain (x1 = y1 = 0: the
+ * whole domain, the only one the unsplit schemes take), the launch shape and
+ * a device of `cu_count` CUs.  For every wave of the grid, 3 ints in `segs`
  * (capacity `max_waves` waves): strip, first row and row count relative to
  * the sub-area, or (-1, 0, 0) for a wave with nothing to do; *n_waves receives
  * the grid's wave count.  No counterpart in the reference (its tile grid is

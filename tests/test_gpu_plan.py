@@ -40,3 +40,36 @@ def test_round_plan_equals_equal_segments(ctx, scheme, key, nx, ny, bc, math):
         assert np.array_equal(x, y), (k, float(np.abs(x - y).max()))
     assert a.computeDt() == b.computeDt()
     assert strips > 0
+
+
+@pytest.mark.parametrize("key,order,math", [("lxf", 1, "fast"), ("hll", 1, "exact"), ("force", 1, "fast"),
+                                             ("kp07u", 2, "fast"), ("kp07u", 2, "exact")])
+def test_unsplit_round_plan_equals_equal_segments(ctx, key, order, math):
+    from finitevolumegpu_amd.FORCE import FORCE
+    from finitevolumegpu_amd.HLL import HLL
+    from finitevolumegpu_amd.KP07 import KP07
+    from finitevolumegpu_amd.LxF import LxF
+    from finitevolumegpu_amd.Simulator import BoundaryCondition
+    nx, ny = 4096, 1536
+    segs = plan(key, nx, ny, cu_count=ctx.cu_count)
+    if key in ("hll", "kp07u"):
+        assert len(segs) == 4 * ctx.cu_count * 4, "round plan expected"
+    print("case unsplit round plan", key, math, flush=True)
+    fields = state("kp07_dimsplit", nx, ny, seed=5)
+    bc = BoundaryCondition(BoundaryCondition.getTypes(O.pack_bc(R, P, P, R)))
+
+    def make(**kw):
+        if key == "kp07u":
+            return KP07(ctx, *fields, nx, ny, 1.0, 1.0, 9.81, order=order, boundary_conditions=bc, math=math, **kw)
+        cls = {"lxf": LxF, "hll": HLL, "force": FORCE}[key]
+        return cls(ctx, *fields, nx, ny, 1.0, 1.0, 9.81, boundary_conditions=bc, math=math, **kw)
+
+    a = make()
+    b = make(tile_rows=64, waves_per_block=4)
+    dt = np.float32(0.5 * a.computeDt())
+    for _ in range(3):
+        a.step(dt)
+        b.step(dt)
+    for k, (x, y) in enumerate(zip(a.download(), b.download())):
+        assert np.array_equal(x, y), (k, float(np.abs(x - y).max()))
+    assert a.computeDt() == b.computeDt()

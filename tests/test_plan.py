@@ -12,7 +12,8 @@ import pytest
 
 from finitevolumegpu_amd import _lib
 
-SCHEMES = {"kp07": 0, "hll2": 1, "waf": 2, "ee2d": 3}
+SCHEMES = {"kp07": 0, "hll2": 1, "waf": 2, "ee2d": 3, "lxf": 4, "hll": 5, "force": 6, "kp07u": 7}
+STRIP = {"lxf": 62, "hll": 62, "force": 62}  # columns per strip (64 lanes less the ghost lanes)
 
 
 def plan(scheme, nx, ny, sub=(0, 0, 0, 0), tile_rows=0, waves_per_block=4, xcd_remap=True, cu_count=256):
@@ -29,8 +30,8 @@ def plan(scheme, nx, ny, sub=(0, 0, 0, 0), tile_rows=0, waves_per_block=4, xcd_r
     return segs
 
 
-def assert_covers(segs, width, height):
-    strips = -(-width // 60)
+def assert_covers(segs, width, height, strip=60):
+    strips = -(-width // strip)
     cover = np.zeros((strips, height), dtype=np.int32)
     for s, r, c in segs:
         if s < 0:
@@ -57,6 +58,12 @@ CASES = [
     ("hll2", 4096, 4096, (0, 0, 0, 0), 0, 2, False, 256),
     ("kp07", 20000, 300, (0, 0, 0, 0), 0, 4, True, 256),
     ("kp07", 2048, 2048, (0, 0, 0, 0), 0, 4, True, 80),
+    ("lxf", 4096, 4096, (0, 0, 0, 0), 0, 4, True, 256),
+    ("hll", 3000, 2500, (0, 0, 0, 0), 0, 4, True, 256),
+    ("force", 777, 333, (0, 0, 0, 0), 0, 4, True, 256),
+    ("kp07u", 4096, 4096, (0, 0, 0, 0), 0, 4, True, 256),
+    ("kp07u", 4096, 4096, (0, 0, 0, 0), 16, 2, True, 256),
+    ("lxf", 13, 11, (0, 0, 0, 0), 0, 4, True, 256),
 ]
 
 
@@ -66,7 +73,7 @@ def test_segments_cover_the_sub_area_once(scheme, nx, ny, sub, tr, wpb, remap, c
     x0, y0, x1, y1 = sub
     x1 = x1 or nx
     y1 = y1 or ny
-    assert_covers(segs, x1 - x0, y1 - y0)
+    assert_covers(segs, x1 - x0, y1 - y0, STRIP.get(scheme, 60))
 
 
 def test_round_weighted_plan_at_4096():
@@ -97,8 +104,19 @@ def test_equal_plan_stays_within_four_waves_per_simd():
     assert (segs[:, 0] >= 0).sum() <= 256 * 4 * 4
 
 
+def test_unsplit_round_plan_for_hll_and_kp07_only():
+    for key, strips in (("hll", 67), ("kp07u", 69)):
+        segs = plan(key, 4096, 4096)
+        assert len(segs) == 4 * 256 * 4
+        assert (segs[:, 0] >= 0).sum() == strips * (1024 // strips) * 4
+    for key in ("lxf", "force"):
+        counts = plan(key, 4096, 4096)[:, 2]
+        assert len(np.unique(counts[counts > 0])) <= 2  # equal segments (the last of a strip shorter)
+
+
 def test_plan_rejects_bad_arguments():
     lib = _lib.lib()
     n = ctypes.c_int()
-    assert lib.fvmi_substep_plan(7, 64, 64, 0, 0, 0, 0, None, 256, None, 0, ctypes.byref(n)) < 0
+    assert lib.fvmi_substep_plan(8, 64, 64, 0, 0, 0, 0, None, 256, None, 0, ctypes.byref(n)) < 0
+    assert lib.fvmi_substep_plan(4, 64, 64, 1, 0, 0, 0, None, 256, None, 0, ctypes.byref(n)) < 0
     assert lib.fvmi_substep_plan(0, 64, 64, 0, 0, 65, 0, None, 256, None, 0, ctypes.byref(n)) < 0
