@@ -778,7 +778,12 @@ __device__ __forceinline__ void setup_strip(const SubstepArgs& a, Strip& st, int
 // consecutive groups of one XCD's contiguous range of strips.
 constexpr int kRounds = 4;
 constexpr int kRoundWeights[kRounds] = {13, 11, 9, 7};
-constexpr int kRoundWeightSum = 13 + 11 + 9 + 7;
+constexpr int round_weight_sum() {
+    int sum = 0;
+    for (int k = 0; k < kRounds; ++k) sum += kRoundWeights[k];
+    return sum;
+}
+constexpr int kRoundWeightSum = round_weight_sum();
 
 // rows [lo, hi) of part k of a group of n rows
 __host__ __device__ inline void round_part(int n, int k, int& lo, int& hi) {
@@ -818,7 +823,8 @@ __host__ __device__ inline bool round_segment_of(int block, int wave, int xcds, 
 // whose shortest part has at least `min_part` rows.
 __host__ __device__ inline bool round_split_fits(int strips, int rows, int waves_per_block, int cu_count, int min_part,
                                                  int& gps, int& grows) {
-    if (waves_per_block != kRounds || cu_count <= 0 || cu_count % 8 != 0 || strips <= 0) return false;
+    // one 4-wave workgroup per CU and round: a wave on every SIMD
+    if (waves_per_block != 4 || cu_count <= 0 || cu_count % 8 != 0 || strips <= 0) return false;
     gps = (cu_count * waves_per_block) / strips;
     if (gps <= 0) return false;
     grows = (rows + gps - 1) / gps;
