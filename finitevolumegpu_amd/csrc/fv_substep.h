@@ -338,20 +338,21 @@ __device__ __forceinline__ void to_y(const float q[NV], float o[NV]) {
 }
 
 // Fast-policy reconstruction + half-step predictor of one cell from its
-// backward and forward differences b = c - l, f = r - c:
+// backward and forward differences, b(k) = c - l and f(k) = r - c of variable
+// k (callables, so that each difference is formed where it is used):
 //   minmod(theta b, (b + f)/2, theta f) = theta minmod(b, (b + f)/(2 theta), f)
 // for theta > 0, so each neighbour enters once (a cross-lane operand in the x
 // sweep) and the y sweep reuses the forward difference of one row as the
 // backward difference of the next.
-template <class S>
-__device__ __forceinline__ void reconstruct_fast(const float b[S::NV], const float f[S::NV], const float c[S::NV],
-                                                 const SubstepArgs& a, float kk, float qbl[S::NV],
-                                                 float qbr[S::NV]) {
+template <class S, class B, class F>
+__device__ __forceinline__ void reconstruct_fast(B b, F f, const float c[S::NV], const SubstepArgs& a, float kk,
+                                                 float qbl[S::NV], float qbr[S::NV]) {
     constexpr int NV = S::NV;
     float QL[NV], QR[NV], D[NV], m[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-        m[k] = __builtin_amdgcn_fmed3f(__builtin_amdgcn_fmed3f(b[k], f[k], 0.0f), (b[k] + f[k]) * a.inv_2theta, 0.0f);
+        const float bk = b(k), fk = f(k);
+        m[k] = __builtin_amdgcn_fmed3f(__builtin_amdgcn_fmed3f(bk, fk, 0.0f), (bk + fk) * a.inv_2theta, 0.0f);
         QL[k] = __builtin_fmaf(-a.half_theta, m[k], c[k]);
         QR[k] = __builtin_fmaf(a.half_theta, m[k], c[k]);
     }
@@ -384,21 +385,8 @@ __device__ __forceinline__ void reconstruct(const float l[S::NV], const float c[
             qbr[k] = QR[k] + kk * D[k];
         }
     } else {
-        // as reconstruct_fast with b = c - l, f = r - c
-        float QL[NV], QR[NV], D[NV], m[NV];
-#pragma unroll
-        for (int k = 0; k < NV; ++k) {
-            const float b = c[k] - l[k], f = r[k] - c[k];
-            m[k] = __builtin_amdgcn_fmed3f(__builtin_amdgcn_fmed3f(b, f, 0.0f), (b + f) * a.inv_2theta, 0.0f);
-            QL[k] = __builtin_fmaf(-a.half_theta, m[k], c[k]);
-            QR[k] = __builtin_fmaf(a.half_theta, m[k], c[k]);
-        }
-        S::pred_diff_fast(QL, QR, c, m, a, D);
-#pragma unroll
-        for (int k = 0; k < NV; ++k) {
-            qbl[k] = QL[k] + kk * D[k];
-            qbr[k] = QR[k] + kk * D[k];
-        }
+        reconstruct_fast<S>([&](int k) { return c[k] - l[k]; }, [&](int k) { return r[k] - c[k]; }, c, a, kk, qbl,
+                            qbr);
     }
 }
 
@@ -566,7 +554,8 @@ __device__ __forceinline__ void slope_row(const SubstepArgs& a, const Strip& st,
         float f[NV];
 #pragma unroll
         for (int k = 0; k < NV; ++k) f[k] = y2[k] - y1[k];
-        reconstruct_fast<S>(dq, f, y1, a, kky, qbb, qbt_next);  // faces of row r-1
+        reconstruct_fast<S>([&](int k) { return dq[k]; }, [&](int k) { return f[k]; }, y1, a, kky, qbb,
+                            qbt_next);  // faces of row r-1
 #pragma unroll
         for (int k = 0; k < NV; ++k) dq[k] = f[k];
     }
