@@ -101,6 +101,23 @@ class SHMEMGrid:
         return np.sort(np.array(factors))[::-1]
 
 
+def exchange_extents(nx, ny, gx, gy):
+    """[x0, y0, width, height] of the regions of a padded (ny + 2 gy) x (nx + 2 gx)
+    subdomain that the exchange reads (interior cells next to each edge) and
+    writes (the ghost cells beyond it), in the order east, west, north, south
+    (SHMEMSimulator.py:124-136).  East/west regions span the full padded height,
+    so they carry the corners filled by the north/south leg."""
+    read_e = np.array([nx, 0, gx, ny + 2 * gy])
+    read_w = np.array([gx, 0, gx, ny + 2 * gy])
+    read_n = np.array([gx, ny, nx, gy])
+    read_s = np.array([gx, gy, nx, gy])
+    write_e = read_e + np.array([gx, 0, 0, 0])
+    write_w = read_w - np.array([gx, 0, 0, 0])
+    write_n = read_n + np.array([0, gy, 0, 0])
+    write_s = read_s - np.array([0, gy, 0, 0])
+    return {"read": (read_e, read_w, read_n, read_s), "write": (write_e, write_w, write_n, write_s)}
+
+
 class SHMEMSimulator(BaseSimulator):
     """Steps a group of subdomain simulators as one (SHMEMSimulator.py:31-263).
 
@@ -161,16 +178,9 @@ class SHMEMSimulator(BaseSimulator):
         self.nvars = len(s0.getOutput().gpu_variables)
         a0 = s0.getOutput()[0]
         self.gc_x, self.gc_y = int(a0.x_halo), int(a0.y_halo)
-        nx, ny, gx, gy = int(self.nx), int(self.ny), self.gc_x, self.gc_y
-        # [x0, y0, width, height] of the regions read and written (SHMEMSimulator.py:124-136)
-        self.read_e = np.array([nx, 0, gx, ny + 2 * gy])
-        self.read_w = np.array([gx, 0, gx, ny + 2 * gy])
-        self.read_n = np.array([gx, ny, nx, gy])
-        self.read_s = np.array([gx, gy, nx, gy])
-        self.write_e = self.read_e + np.array([gx, 0, 0, 0])
-        self.write_w = self.read_w - np.array([gx, 0, 0, 0])
-        self.write_n = self.read_n + np.array([0, gy, 0, 0])
-        self.write_s = self.read_s - np.array([0, gy, 0, 0])
+        ext = exchange_extents(int(self.nx), int(self.ny), self.gc_x, self.gc_y)
+        self.read_e, self.read_w, self.read_n, self.read_s = ext["read"]
+        self.write_e, self.write_w, self.write_n, self.write_s = ext["write"]
         self.exchange_stream = Stream()
         self._done = [Event() for _ in sims]
         self._exchanged = Event()

@@ -75,3 +75,35 @@ def test_region_copy_of_nothing_launches_nothing():
     # empty regions are skipped before any launch
     assert lib.fvmi_copy_regions_f32(_regions((0x1000, 0x2000, 256, 256, 0, 7)), 1, None) == 0
     assert lib.fvmi_copy_regions_f32(None, -1, None) < 0
+
+
+@pytest.mark.parametrize("nx,ny,g", [(16, 12, 2), (5, 9, 1), (40, 24, 2)])
+def test_exchange_extents(nx, ny, g):
+    """Writes cover the ghost frame exactly once per leg (north/south: the ghost
+    rows over the interior columns; east/west: the ghost columns over the full
+    height); reads are interior cells, the neighbour's cells next to the shared
+    edge; every region lies inside the padded array."""
+    from finitevolumegpu_amd.SHMEMSimulator import exchange_extents
+    ext = exchange_extents(nx, ny, g, g)
+    H, W = ny + 2 * g, nx + 2 * g
+
+    def mask(r):
+        x, y, w, h = (int(v) for v in r)
+        assert 0 <= x and 0 <= y and x + w <= W and y + h <= H
+        m = np.zeros((H, W), dtype=int)
+        m[y:y + h, x:x + w] = 1
+        return m
+
+    read_e, read_w, read_n, read_s = ext["read"]
+    write_e, write_w, write_n, write_s = ext["write"]
+    interior = np.zeros((H, W), dtype=int)
+    interior[g:g + ny, g:g + nx] = 1
+    ns = mask(write_n) + mask(write_s)
+    assert ns.max() == 1 and (ns * interior).sum() == 0 and ns.sum() == 2 * g * nx
+    ew = mask(write_e) + mask(write_w)
+    assert ew.max() == 1 and (ew * interior).sum() == 0 and ew.sum() == 2 * g * H
+    # reads: interior columns (all rows for east/west), interior rows for north/south
+    assert (mask(read_n) * interior).sum() == g * nx and (mask(read_s) * interior).sum() == g * nx
+    assert mask(read_e)[:, nx:nx + g].sum() == g * H and mask(read_w)[:, g:2 * g].sum() == g * H
+    # a subdomain's north ghost rows receive its northern neighbour's southmost interior rows
+    assert read_s[1] == g and write_n[1] == ny + g and read_n[1] == ny and write_s[1] == 0
