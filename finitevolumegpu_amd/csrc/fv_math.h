@@ -234,13 +234,45 @@ __device__ __forceinline__ float waf_hstar(const WafCell& l, const WafCell& r, f
     }
 }
 
-// WAF_1D_flux (SWECommon.h:131-210) between cells l (= Q_l1) and r (= Q_r1),
-// given depth and tangential velocity of the outer cells (Q_l2, Q_r2) and h*
-// of the interfaces (l2, l), (l, r), (r, r2).  dtd = dt / d (fast policy).
+// desingularize(x, 1e-6) of the WAF limiter ratios (SWECommon.h:170-176); fast
+// form: the sign of x on the magnitude max(|x|, min(x^2/(2 eps) + eps/2, eps))
 template <class M>
-__device__ __forceinline__ void waf_flux_core(const WafCell& L, const WafCell& R, float h_l2, float v_l2, float h_r2,
-                                              float v_r2, float h_dag_l, float h_dag, float h_dag_r, float g,
-                                              float hg, float d, float dt, float dtd, float out[3]) {
+__device__ __forceinline__ float waf_desing(float x) {
+    if constexpr (M::exact) {
+        return desingularize<M>(x, 1.0e-6f);
+    } else {
+        const float eps = 1.0e-6f, inv_2eps = 0.5e6f;
+        return __builtin_copysignf(
+            __builtin_fmaxf(__builtin_fabsf(x), __builtin_fminf(x * x * inv_2eps + 0.5f * eps, eps)), x);
+    }
+}
+
+// The limiter denominators of the interface between cells l and r with h* hd:
+// desingularize(h* - h_l), desingularize(v_r - v_l), desingularize(h_r - h*)
+// (SWECommon.h:170-176).  The upwind numerators of an interface's three
+// ratios are the same quantities of the interface to its left (positive
+// Courant number) or to its right, with the same operands in the same order,
+// so each interface computes its own once and its neighbours take them.
+struct WafDen {
+    float d1, d2, d3;
+};
+
+template <class M>
+__device__ __forceinline__ WafDen waf_den(const WafCell& l, const WafCell& r, float hd) {
+    WafDen D;
+    D.d1 = waf_desing<M>(hd - l.h);
+    D.d2 = waf_desing<M>(r.v - l.v);
+    D.d3 = waf_desing<M>(r.h - hd);
+    return D;
+}
+
+// WAF_1D_flux (SWECommon.h:131-210) between cells l (= Q_l1) and r (= Q_r1)
+// with h* h_dag, given the limiter denominators of this interface (D) and of
+// the interfaces to its left (Dl) and right (Dr).  dtd = dt / d (fast policy).
+template <class M>
+__device__ __forceinline__ void waf_flux_core(const WafCell& L, const WafCell& R, const WafDen& Dl, const WafDen& D,
+                                              const WafDen& Dr, float h_dag, float g, float hg, float d, float dt,
+                                              float dtd, float out[3]) {
     const float h_l = L.h, h_r = R.h, u_l = L.u, u_r = R.u, v_l = L.v, v_r = R.v, c_l = L.c, c_r = R.c;
     const float q_l1[3] = {L.h, L.hu, L.hv};
     const float q_r1[3] = {R.h, R.hu, R.hv};
@@ -268,13 +300,10 @@ __device__ __forceinline__ void waf_flux_core(const WafCell& L, const WafCell& R
         const float c_1 = S_l * dt / d;
         const float c_2 = S_star * dt / d;
         const float c_3 = S_r * dt / d;
-        const float eps = 1.0e-6f;
-        const float r_1 = M::div(desingularize<M>((c_1 > 0.0f) ? (h_dag_l - h_l2) : (h_dag_r - h_r), eps),
-                                 desingularize<M>(h_dag - h_l, eps));
-        const float r_2 = M::div(desingularize<M>((c_2 > 0.0f) ? (v_l - v_l2) : (v_r2 - v_r), eps),
-                                 desingularize<M>(v_r - v_l, eps));
-        const float r_3 = M::div(desingularize<M>((c_3 > 0.0f) ? (h_l - h_dag_l) : (h_r2 - h_dag_r), eps),
-                                 desingularize<M>(h_r - h_dag, eps));
+        // desingularize((c_1 > 0) ? h_dag_l - h_l2 : h_dag_r - h_r) / desingularize(h_dag - h_l), ...
+        const float r_1 = M::div((c_1 > 0.0f) ? Dl.d1 : Dr.d1, D.d1);
+        const float r_2 = M::div((c_2 > 0.0f) ? Dl.d2 : Dr.d2, D.d2);
+        const float r_3 = M::div((c_3 > 0.0f) ? Dl.d3 : Dr.d3, D.d3);
         const float A_1 = __builtin_copysignf(1.0f, c_1) * limiter_to_waf(generalized_minmod(r_1, 1.9f), c_1);
         const float A_2 = __builtin_copysignf(1.0f, c_2) * limiter_to_waf(generalized_minmod(r_2, 1.9f), c_2);
         const float A_3 = __builtin_copysignf(1.0f, c_3) * limiter_to_waf(generalized_minmod(r_3, 1.9f), c_3);
@@ -300,21 +329,12 @@ __device__ __forceinline__ void waf_flux_core(const WafCell& L, const WafCell& R
         const float Qsl[3] = {s_l, s_l * S_star, s_l * v_l};
         const float Qsr[3] = {s_r, s_r * S_star, s_r * v_r};
         const float c_1 = S_l * dtd, c_2 = S_star * dtd, c_3 = S_r * dtd;
-        const float eps = 1.0e-6f, inv_2eps = 0.5e6f;
-        // desingularize: the sign of x on the magnitude max(|x|, min(x^2/(2 eps) + eps/2, eps))
-        auto desing = [&](float x) {
-            return __builtin_copysignf(
-                __builtin_fmaxf(__builtin_fabsf(x), __builtin_fminf(x * x * inv_2eps + 0.5f * eps, eps)), x);
-        };
         auto gmm = [](float r) {
             return __builtin_fmaxf(0.0f, __builtin_fminf(1.9f * r, __builtin_fminf((1.0f + r) * 0.5f, 1.9f)));
         };
-        const float n1 = desing((c_1 > 0.0f) ? (h_dag_l - h_l2) : (h_dag_r - h_r));
-        const float n2 = desing((c_2 > 0.0f) ? (v_l - v_l2) : (v_r2 - v_r));
-        const float n3 = desing((c_3 > 0.0f) ? (h_l - h_dag_l) : (h_r2 - h_dag_r));
-        const float r_1 = n1 * M::rcp(desing(h_dag - h_l));
-        const float r_2 = n2 * M::rcp(desing(v_r - v_l));
-        const float r_3 = n3 * M::rcp(desing(h_r - h_dag));
+        const float r_1 = ((c_1 > 0.0f) ? Dl.d1 : Dr.d1) * M::rcp(D.d1);
+        const float r_2 = ((c_2 > 0.0f) ? Dl.d2 : Dr.d2) * M::rcp(D.d2);
+        const float r_3 = ((c_3 > 0.0f) ? Dl.d3 : Dr.d3) * M::rcp(D.d3);
         const float A_1 = __builtin_copysignf(1.0f, c_1) * (1.0f - (1.0f - __builtin_fabsf(c_1)) * gmm(r_1));
         const float A_2 = __builtin_copysignf(1.0f, c_2) * (1.0f - (1.0f - __builtin_fabsf(c_2)) * gmm(r_2));
         const float A_3 = __builtin_copysignf(1.0f, c_3) * (1.0f - (1.0f - __builtin_fabsf(c_3)) * gmm(r_3));

@@ -418,13 +418,15 @@ __device__ __forceinline__ void x_sweep_row(float q[S::NV], const SubstepArgs& a
         for (int k = 0; k < NV; ++k) qbl_r[k] = from_right(qbl[k]);
         S::flux_x(qbr, qbl_r, a, F);  // interface (c, c+1)
     } else {
-        // WAF: each lane's cell quantities and the h* of its right interface
-        // once; neighbours' through DPP (Q_l2 = c-1, Q_r1 = c+1, Q_r2 = c+2)
+        // WAF: each lane's cell quantities, and the h* and limiter denominators
+        // of its right interface, once; the neighbours' through DPP
         const WafCell P = waf_cell<M>(q, a.g, a.sqrt_g);
         const WafCell R = waf_shift_right(P);
         const float hd = waf_hstar<M>(P, R, a.g, a.inv_g);
-        waf_flux_core<M>(P, R, from_left(P.h), from_left(P.v), from_right(R.h), from_right(R.v), from_left(hd), hd,
-                         from_right(hd), a.g, a.hg, a.dx, a.dt, a.dtdx, F);
+        const WafDen D = waf_den<M>(P, R, hd);
+        const WafDen Dl = {from_left(D.d1), from_left(D.d2), from_left(D.d3)};
+        const WafDen Dr = {from_right(D.d1), from_right(D.d2), from_right(D.d3)};
+        waf_flux_core<M>(P, R, Dl, D, Dr, hd, a.g, a.hg, a.dx, a.dt, a.dtdx, F);
     }
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
@@ -587,17 +589,17 @@ __device__ __forceinline__ void slope_row(const SubstepArgs& a, const Strip& st,
     }
 }
 
-// WAF: the y window holds 4 rows (r-3 .. r) and there is no slope state.
+// WAF: the y window holds 3 rows (r-2 .. r) and there is no slope state.
 // Each ring slot carries its row's WafCell (sweep-y view), computed once when
-// the row is ready; hd[0], hd[1] carry h* of the interfaces (r-3, r-2) and
-// (r-2, r-1); g is the flux through the bottom interface of row r-2.  All
-// carried values are replaced in place.
+// the row is ready; hd carries h* of the interface (r-2, r-1) and dd[0], dd[1]
+// the limiter denominators of the interfaces (r-3, r-2) and (r-2, r-1); g is
+// the flux through the bottom interface of row r-2.  All carried values are
+// replaced in place.
 template <class S, class M, bool XFirst>
-__device__ __forceinline__ void waf_row(const SubstepArgs& a, const Strip& st, const float (&w0)[3],
-                                        const float (&w1)[3], float (&w3)[3], const WafCell& c0, const WafCell& c1,
-                                        const WafCell& c2, WafCell& c3, float (&hd)[2], float (&g)[3], int r,
-                                        bool emit, float& dt_min) {
-    (void)w0;
+__device__ __forceinline__ void waf_row(const SubstepArgs& a, const Strip& st, const float (&w1)[3],
+                                        float (&w3)[3], const WafCell& c1, const WafCell& c2, WafCell& c3,
+                                        float& hd, WafDen (&dd)[2], float (&g)[3], int r, bool emit, float& dt_min) {
+    // w1: row r-2 (x swept when XFirst), w3: raw row r on entry
     if constexpr (XFirst) x_sweep_row<S, M>(w3, a);
     {
         float y3[3];
@@ -605,8 +607,9 @@ __device__ __forceinline__ void waf_row(const SubstepArgs& a, const Strip& st, c
         c3 = waf_cell<M>(y3, a.g, a.sqrt_g);
     }
     const float hd_r = waf_hstar<M>(c2, c3, a.g, a.inv_g);  // interface (r-1, r)
+    const WafDen Dn = waf_den<M>(c2, c3, hd_r);
     float G[3];
-    waf_flux_core<M>(c1, c2, c0.h, c0.v, c3.h, c3.v, hd[0], hd[1], hd_r, a.g, a.hg, a.dy, a.dt, a.dtdy, G);
+    waf_flux_core<M>(c1, c2, dd[0], dd[1], Dn, hd, a.g, a.hg, a.dy, a.dt, a.dtdy, G);  // interface (r-2, r-1)
     if (emit) {
         float y1[3], outq[3];
         to_y<3>(w1, y1);
@@ -619,8 +622,9 @@ __device__ __forceinline__ void waf_row(const SubstepArgs& a, const Strip& st, c
         // computeDt reads after a full step (the last launch's values)
         if constexpr (!XFirst) accumulate_cfl<S, M>(outq, a, st.lane_out, dt_min);
     }
-    hd[0] = hd[1];
-    hd[1] = hd_r;
+    hd = hd_r;
+    dd[0] = dd[1];
+    dd[1] = Dn;
 #pragma unroll
     for (int k = 0; k < 3; ++k) g[k] = G[k];
 }
@@ -689,16 +693,19 @@ __device__ __forceinline__ float march_slopes(const SubstepArgs& a, const Strip&
     return dt_min;
 }
 
-// Row march of WAF (4-row window): row m and its WafCell in ring slot m % N.
+// Row march of WAF (3-row window): row m and its WafCell in ring slot m % N.
 template <class S, class M, bool XFirst>
 __device__ __forceinline__ float march_waf(const SubstepArgs& a, const Strip& st) {
     constexpr int PF = S::kPrefetch;
-    constexpr int N = PF + 4;  // rows r-3 .. r and PF rows in flight
+    constexpr int N = PF + 3;  // rows r-2 .. r and PF rows in flight
     float dt_min = M::exact ? __builtin_inff() : 0.0f;
-    float ring[N][3], G[3], hd[2];
+    float ring[N][3], G[3], hd;
     WafCell cell[N];
+    WafDen dd[2];
     unsigned ys[N];  // y-momentum sign mask of each slot's row
-    const int last = st.r_end - 1;
+    const int last = st.r_end - 1;  // r_end >= 5
+    // prologue: rows 0 .. N-1; interfaces (0, 1) and (1, 2); row 3 + PF; then
+    // step r = 3 (flux (1,2), no output)
 #pragma unroll
     for (int m = 0; m < N; ++m) ys[m] = load_row_raw<3, S::kPinRows>(a, st, min(m, last), ring[m]);
 #pragma unroll
@@ -709,18 +716,25 @@ __device__ __forceinline__ float march_waf(const SubstepArgs& a, const Strip& st
         to_y<3>(ring[m], y);
         cell[m] = waf_cell<M>(y, a.g, a.sqrt_g);
     }
-    hd[0] = waf_hstar<M>(cell[0], cell[1], a.g, a.inv_g);  // interface (0, 1)
-    hd[1] = waf_hstar<M>(cell[1], cell[2], a.g, a.inv_g);  // interface (1, 2)
+    {
+        const float hd01 = waf_hstar<M>(cell[0], cell[1], a.g, a.inv_g);
+        hd = waf_hstar<M>(cell[1], cell[2], a.g, a.inv_g);
+        dd[0] = waf_den<M>(cell[0], cell[1], hd01);
+        dd[1] = waf_den<M>(cell[1], cell[2], hd);
+    }
 #pragma unroll
     for (int k = 0; k < 3; ++k) G[k] = 0.0f;
+    // row 3 + PF into the slot of row 0, whose last use was above
+    ys[(3 + PF) % N] = load_row_raw<3, S::kPinRows>(a, st, min(3 + PF, last), ring[(3 + PF) % N]);
     use_row<3, S::kPinRows>(st, ys[3], ring[3]);
-    waf_row<S, M, XFirst>(a, st, ring[0], ring[1], ring[3], cell[0], cell[1], cell[2], cell[3], hd, G, 3, false,
-                          dt_min);  // flux (1,2)
+    waf_row<S, M, XFirst>(a, st, ring[1], ring[3], cell[1], cell[2], cell[3], hd, dd, G, 3, false, dt_min);
+    // steady state: r = 4, 4 + N, ...; the step of row rr loads row rr + PF into
+    // the slot of row rr - 3, whose last use was the previous step
     auto step = [&](int u, int rr) {
         ys[(4 + u + PF) % N] = load_row_raw<3, S::kPinRows>(a, st, min(rr + PF, last), ring[(4 + u + PF) % N]);
         use_row<3, S::kPinRows>(st, ys[(4 + u) % N], ring[(4 + u) % N]);
-        waf_row<S, M, XFirst>(a, st, ring[(1 + u) % N], ring[(2 + u) % N], ring[(4 + u) % N], cell[(1 + u) % N],
-                              cell[(2 + u) % N], cell[(3 + u) % N], cell[(4 + u) % N], hd, G, rr, true, dt_min);
+        waf_row<S, M, XFirst>(a, st, ring[(2 + u) % N], ring[(4 + u) % N], cell[(2 + u) % N], cell[(3 + u) % N],
+                              cell[(4 + u) % N], hd, dd, G, rr, true, dt_min);
     };
     int r = 4;
     for (; r + N <= st.r_end; r += N) {
