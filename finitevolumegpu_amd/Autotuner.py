@@ -12,7 +12,10 @@ rows per 64-lane strip (``tile_rows``) and strips per workgroup
 
 ``ensemble_schedule`` extends it to the batched engine: it times candidate
 segment heights and workgroup sizes on the actual instance mix and returns
-the fastest (``tile_rows`` 0 and ``seg_rows`` None mean the balanced plan).
+the fastest.  ``tile_rows`` 0 is the library's own plan (the round-weighted
+split where it fits, else equal segments balanced over the device); the tuned
+shape is applied at every grid size, so a tuned 0 keeps that choice per size.
+``seg_rows`` None is the ensemble's balanced plan.
 """
 
 import logging
