@@ -218,7 +218,7 @@ static_assert(kRowPrefetch >= 1 && kRowPrefetch <= 3, "row prefetch distance 1..
 // kPinRows: the march pins each prefetched row at its first use (use_row);
 // kCarryDiff: the y sweep of the fast policy carries one row's forward
 // difference over as the next row's backward difference; kPrefetch: rows
-// loaded ahead (kRowPrefetch).  Measured at 4096^2 (SWE) and 8192^2 (Euler):
+// loaded ahead (kRowPrefetch unless a scheme sets its own).  Measured at 4096^2 (SWE) and 8192^2 (Euler):
 // pinning takes the WAF substep from 210 to 197 us and HLL2 about 1 % faster,
 // KP07_dimsplit even; the Euler kernel is 7 % (carry) and 20-25 % (pinned rows)
 // slower with them, so it keeps neither, and 1.5 % faster with one row ahead
@@ -251,6 +251,7 @@ struct SweBase {
 
 template <class M>
 struct SweKP07 : SweBase<M> {
+    static constexpr int kPrefetch = 3;  // 0.7 % faster than 2 at 4096^2
     static constexpr bool kSlopes = true;
     __device__ static void flux(const float l[3], const float r[3], const SubstepArgs& a, float o[3]) {
         if constexpr (M::exact)
@@ -264,6 +265,7 @@ struct SweKP07 : SweBase<M> {
 
 template <class M>
 struct SweHLL2 : SweBase<M> {
+    static constexpr int kPrefetch = 3;  // 1.1 % faster than 2 at 4096^2
     static constexpr bool kSlopes = true;
     __device__ static void flux(const float l[3], const float r[3], const SubstepArgs& a, float o[3]) {
         if constexpr (M::exact)
