@@ -1,0 +1,111 @@
+"""Supersonic shallow-water states on the GPU against the float32 oracle.
+
+The HLL flux takes F_l when S_l >= 0, F_r when S_r <= 0 and the HLL average
+otherwise (SWECommon.h:299-311); the fast policy folds that selection into
+clamped wave speeds.  The smooth random states of the other parity tests stay
+in the average.  Here streams at Froude number ~2.5 in x and ~1.9 in y take
+both upwind branches, and where they meet head-on S_r <= 0 <= S_l (the
+reference tests S_l first).  Outflow edges keep the streams from wrapping
+round into a rarefaction that would dry the bed.
+
+* math='exact': bitwise equality of every field and of the CFL minimum.
+* math='fast' : relative L1 <= max(1e-5, 2 x the float32-vs-float64 oracle
+  difference) per field.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import fv_oracle as O
+from oracle import fv_oracle_unsplit as U
+from tests._cases import F, ghost_frame, install_frame, make_gpu_sim, make_oracle
+from tests.test_gpu_unsplit import _frame, _gpu
+
+pytestmark = pytest.mark.gpu
+
+FAST_TOL = 1e-5
+BC = O.pack_bc(F, F, F, F)
+NX, NY, STEPS = 131, 77, 10
+
+
+def supersonic_state(nx, ny, seed):
+    """h ~ 1; u = +8 left of nx/2 and -8 right of it; v = +6 below ny/2 and -6
+    above it (sqrt(g h) ~ 3.1)."""
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[0:ny, 0:nx] + 0.5
+    h = 1.0 + 0.1 * np.sin(2.0 * np.pi * x / nx) * np.cos(2.0 * np.pi * y / ny) + rng.uniform(-1e-3, 1e-3, (ny, nx))
+    u = np.where(x < 0.5 * nx, 8.0, -8.0)
+    v = np.where(y < 0.5 * ny, 6.0, -6.0)
+    return [np.ascontiguousarray(a, dtype=np.float32) for a in (h, h * u, h * v)]
+
+
+def _tolerance(got, ref, ref64):
+    for k, (a, b) in enumerate(zip(got, ref)):
+        err = O.rel_l1(a, b)
+        floor = O.rel_l1(b, ref64[k])
+        print("field", k, "fast-vs-f32", err, "f32-vs-f64", floor, flush=True)
+        assert np.all(np.isfinite(a)), k
+        assert err <= max(FAST_TOL, 2.0 * floor), (k, err, floor)
+
+
+@pytest.mark.parametrize("scheme", ["kp07_dimsplit", "hll2", "waf"])
+@pytest.mark.parametrize("math", ["exact", "fast"])
+def test_dimsplit_supersonic(ctx, scheme, math):
+    print("case", scheme, math, flush=True)
+    fields = supersonic_state(NX, NY, 5)
+    sim = make_gpu_sim(ctx, scheme, fields, NX, NY, 1.0, 1.0, BC, math=math)
+    ora = make_oracle(scheme, fields, NX, NY, 1.0, 1.0, BC)
+    padded = ghost_frame(fields, 3)
+    install_frame(sim, ora, padded)
+    dt = np.float32(0.45 * float(ora.cfl))
+    assert np.float32(sim.computeDt() * 2.0) == ora.cfl
+    ora64 = None
+    if math == "fast":
+        ora64 = make_oracle(scheme, fields, NX, NY, 1.0, 1.0, BC, dtype=np.float64)
+        for k, a in enumerate(padded):
+            ora64.u0[k][:] = a
+            ora64.u1[k][:] = a
+    for _ in range(STEPS):
+        sim.step(dt)
+        ora.step(dt)
+        if ora64 is not None:
+            ora64.step(dt)
+    got, ref = sim.download(), ora.download()
+    assert float(ref[0].max()) > 1.5  # the collisions have built up
+    if math == "exact":
+        for k, (a, b) in enumerate(zip(got, ref)):
+            assert np.array_equal(a, b), "field %d: max |diff| %g" % (k, np.abs(a - b).max())
+        if scheme != "waf":
+            assert np.float32(sim.computeDt() * 2.0) == ora.cfl
+    else:
+        _tolerance(got, ref, ora64.download())
+
+
+@pytest.mark.parametrize("math", ["exact", "fast"])
+def test_first_order_hll_supersonic(ctx, math):
+    print("case hll", math, flush=True)
+    fields = supersonic_state(NX, NY, 6)
+    sim = _gpu(ctx, "hll", fields, NX, NY, BC, math)
+    oras = [U.OracleUnsplitSimulator("hll", fields, NX, NY, 1.0, 1.0, 9.81, bc=BC, dtype=dt)
+            for dt in ((np.float32,) if math == "exact" else (np.float32, np.float64))]
+    gc = U.SCHEME_GC["hll"]
+    padded = _frame(fields, gc)
+    for k, a in enumerate(padded):
+        for buf in (sim.u0, sim.u1):
+            buf[k].upload(sim.stream, a, extent=[0, 0, NX + 2 * gc, NY + 2 * gc])
+        for ora in oras:
+            ora.u0[k][:] = a
+            ora.u1[k][:] = a
+    sim.synchronize()
+    dt = np.float32(0.3 * float(oras[0].cfl))
+    for _ in range(STEPS):
+        sim.step(dt)
+        for ora in oras:
+            ora.step(dt)
+    got, ref = sim.download(), oras[0].download()
+    assert float(ref[0].max()) > 1.5
+    if math == "exact":
+        for k, (a, b) in enumerate(zip(got, ref)):
+            assert np.array_equal(a, b), "field %d: max |diff| %g" % (k, np.abs(a - b).max())
+    else:
+        _tolerance(got, ref, oras[1].download())
