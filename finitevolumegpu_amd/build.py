@@ -3,10 +3,10 @@
     python -m finitevolumegpu_amd.build [--verbose] [--define NAME=VALUE ...] [--flag FLAG ...] [--out PATH]
 
 Four translation units: the exact-arithmetic kernels (-ffp-contract=off, IEEE
-division/square root), the fast kernels (contraction allowed; the shallow-water
-kernels in a unit of their own with the ILP-maximising scheduler, the Euler
-kernels with the default one) and the C-ABI runtime.  Packed FP32 instructions are disabled for
-all device code.
+division/square root), the fast kernels (contraction allowed, the ILP-maximising
+scheduler; the Euler and the shallow-water kernels in units of their own, which
+compile in parallel) and the C-ABI runtime.  Packed FP32 instructions are
+disabled for all device code.
 """
 
 import os
@@ -27,7 +27,9 @@ COMMON = [
 
 UNITS = [
     ("fv_kernels_exact.hip", ["-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt"]),
-    ("fv_kernels_fast.hip", ["-ffp-contract=on"]),
+    # the Euler kernels: 1 % faster with the ILP-maximising scheduler than with
+    # the default one on the EE2D 8192^2 substep
+    ("fv_kernels_fast.hip", ["-ffp-contract=on", "-mllvm", "--amdgpu-sched-strategy=max-ilp"]),
     # the shallow-water kernels with the ILP-maximising scheduler: 4 % faster on the
     # KP07_dimsplit 4096^2 substep, 6 % on its ensemble, 2 % on WAF and LxF
     ("fv_kernels_fast_ilp.hip", ["-ffp-contract=on", "-mllvm", "--amdgpu-sched-strategy=max-ilp"]),

@@ -141,15 +141,10 @@ __device__ __forceinline__ void euler_hll_flux(const float ql[4], const float qr
     const float S_r = R.u + R.c * q_r;
     const float F_l[4] = {ql[1], ql[1] * L.u + L.P, ql[2] * L.u, L.u * (ql[3] + L.P)};
     const float F_r[4] = {qr[1], qr[1] * R.u + R.P, qr[2] * R.u, R.u * (qr[3] + R.P)};
-    // explicit selection: the clamped-speed weights of hll_weights take 8 VALU
-    // instructions per row off this kernel but made its launch 3 % slower
-    const float iden = rcp(S_r - S_l);
-    const float wl = S_r * iden, wr = S_l * iden, wq = S_r * S_l * iden;
+    float wl, wr, wq;
+    hll_weights(S_l, S_r, wl, wr, wq);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const float star = (wl * F_l[k] - wr * F_r[k]) + wq * (qr[k] - ql[k]);
-        out[k] = (S_l >= 0.0f) ? F_l[k] : ((S_r <= 0.0f) ? F_r[k] : star);
-    }
+    for (int k = 0; k < 4; ++k) out[k] = (wl * F_l[k] - wr * F_r[k]) + wq * (qr[k] - ql[k]);
 }
 
 // Central-upwind flux, Euler (EulerCommon.h:171-187)

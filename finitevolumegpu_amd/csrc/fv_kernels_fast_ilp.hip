@@ -2,7 +2,7 @@
 // grid and ensemble; LxF, HLL, FORCE and the unsplit KP07), built with the
 // ILP-maximising instruction scheduler.  Measured at 4096^2 against the default
 // scheduler: KP07_dimsplit 4 % faster, its ensemble 6 %; WAF and LxF 2 % with
-// the pinned row march.  The Euler kernel stays in fv_kernels_fast.hip.
+// the pinned row march.  The Euler kernels are in fv_kernels_fast.hip.
 #include "fv_launch.h"
 #include "fv_unsplit.h"
 
