@@ -76,6 +76,27 @@ struct SubstepArgs {
     float g_theta;     // g theta
 };
 
+// Rows per segment of the equal-segment plan: at most waves_per_simd waves on
+// every SIMD (segments per strip rounded down) and no segment shorter than
+// min_seg rows, so that a segment's halo rows stay a small part of its work.
+// A grid too small for that, where one wave per SIMD would leave each wave at
+// most small_limit rows, is bound by the latency of one wave's march instead:
+// its strips are cut into segments of that length (at least kMinSegRowsSmall
+// rows), at most one wave per SIMD.
+constexpr int kMinSegRowsSmall = 4;
+__host__ __device__ inline int balanced_segment_rows(int rows, int strips, int cu_count, int waves_per_simd,
+                                                     int min_seg, int small_limit) {
+    int k = cu_count * 4 * waves_per_simd / strips;
+    if (k < 1) k = 1;
+    const int seg = (rows + k - 1) / k;
+    if (seg >= min_seg) return seg;
+    int k1 = cu_count * 4 / strips;
+    if (k1 < 1) k1 = 1;
+    const int seg1 = (rows + k1 - 1) / k1;
+    if (seg1 >= min_seg || seg1 > small_limit) return min_seg;
+    return seg1 < kMinSegRowsSmall ? kMinSegRowsSmall : seg1;
+}
+
 // the slope constants of the fast policy from g and theta
 __host__ __device__ inline void set_slope_constants(SubstepArgs& a) {
     a.half_theta = 0.5f * a.theta;
