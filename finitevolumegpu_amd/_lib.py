@@ -125,7 +125,7 @@ SIGNATURES = {
                               + [c_void_p, c_int] * 6 + [c_void_p, c_int, P(LaunchConfig), c_void_p]),
     "fvmi_halo_copy_periodic_x": (c_int, [c_int, c_int, c_int, P(c_void_p), P(c_int), c_void_p]),
     "fvmi_halo_exchange_periodic": (c_int, [c_int, c_int, c_int, P(c_void_p), P(c_int), c_int, c_int, c_void_p]),
-    "fvmi_field_stats": (c_int, [c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "fvmi_field_stats": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "fvmi_ensemble_prepare": (c_int, [c_void_p, c_int, c_void_p]),
     "fvmi_ensemble_substep": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_int, P(LaunchConfig), c_void_p]),
     "fvmi_ensemble_advance": (c_int, [c_void_p, c_int, c_void_p]),

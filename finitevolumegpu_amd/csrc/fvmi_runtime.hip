@@ -251,13 +251,14 @@ __global__ void ensemble_count_kernel(const fvmi_instance_t* table, int n, int* 
     }
 }
 
+// `f` points at interior cell (0, 0); rows are `pitch` bytes apart
 __global__ void field_stats_kernel(int nx, int ny, const float* f, int pitch, double* sum_out, unsigned int* bad_out) {
     double s = 0.0;
     unsigned int bad = 0;
     for (int j = blockIdx.y; j < ny; j += gridDim.y) {
-        const float* row = reinterpret_cast<const float*>(reinterpret_cast<const char*>(f) + (size_t)(j + 2) * pitch);
+        const float* row = reinterpret_cast<const float*>(reinterpret_cast<const char*>(f) + (size_t)j * pitch);
         for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nx; i += gridDim.x * blockDim.x) {
-            const float v = row[i + 2];
+            const float v = row[i];
             if (std::isfinite(v))
                 s += (double)v;
             else
@@ -824,13 +825,16 @@ int fvmi_graph_destroy(void* graph_exec) {
     return FVMI_OK;
 }
 
-int fvmi_field_stats(int nx, int ny, const float* field, int pitch, double* sum_out, unsigned int* nonfinite_out,
-                     void* stream) {
-    if (nx < 1 || ny < 1 || !field || !sum_out || !nonfinite_out || pitch < (nx + 4) * (int)sizeof(float))
+int fvmi_field_stats(int nx, int ny, int x_halo, int y_halo, const float* field, int pitch, double* sum_out,
+                     unsigned int* nonfinite_out, void* stream) {
+    if (nx < 1 || ny < 1 || x_halo < 0 || y_halo < 0 || !field || !sum_out || !nonfinite_out ||
+        (long long)pitch < (long long)(nx + 2 * x_halo) * (long long)sizeof(float))
         return fail(FVMI_ERR_INVALID, "bad field_stats arguments");
     const int bx = (nx + 255) / 256 < 16 ? (nx + 255) / 256 : 16;
     const int by = ny < 4096 ? ny : 4096;
-    hipLaunchKernelGGL(field_stats_kernel, dim3(bx, by), dim3(256), 0, (hipStream_t)stream, nx, ny, field, pitch,
+    const float* interior = reinterpret_cast<const float*>(reinterpret_cast<const char*>(field) +
+                                                           (size_t)y_halo * (size_t)pitch) + x_halo;
+    hipLaunchKernelGGL(field_stats_kernel, dim3(bx, by), dim3(256), 0, (hipStream_t)stream, nx, ny, interior, pitch,
                        sum_out, nonfinite_out);
     FVMI_HIP(hipGetLastError());
     return FVMI_OK;

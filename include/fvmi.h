@@ -217,11 +217,13 @@ int fvmi_halo_exchange_periodic(int nx, int ny, int nvars, float* const* fields,
                                 int wrap_y, void* stream);
 
 /* ArakawaA2D.check (Common.py:772-779): sum of the interior (float64) and the
- * number of non-finite interior values of one field, written to device memory
- * (`sum_out`: 1 double, `nonfinite_out`: 1 unsigned int, both accumulated, so
- * zero them first).  */
-int fvmi_field_stats(int nx, int ny, const float* field, int pitch, double* sum_out, unsigned int* nonfinite_out,
-                     void* stream);
+ * number of non-finite interior values of one field of nx x ny interior cells
+ * with x_halo ghost columns and y_halo ghost rows a side (2 for the
+ * dimensionally split schemes and the unsplit KP07, 1 for LxF, HLL and FORCE),
+ * written to device memory (`sum_out`: 1 double, `nonfinite_out`: 1 unsigned
+ * int, both accumulated, so zero them first).  pitch >= (nx + 2 x_halo) * 4. */
+int fvmi_field_stats(int nx, int ny, int x_halo, int y_halo, const float* field, int pitch, double* sum_out,
+                     unsigned int* nonfinite_out, void* stream);
 
 /* ---- batched ensemble ------------------------------------------------------
  * No counterpart in the reference, whose simulators run one grid per object

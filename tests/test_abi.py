@@ -146,3 +146,17 @@ def test_ensemble_substep_validation():
     assert lib.fvmi_ensemble_substep(0, None, None, None, 0, 0, None, None) == -1
     assert lib.fvmi_ensemble_substep(7, 0x1000, 0x2000, 0x3000, 0, 0, None, None) == -1 and "scheme" in _err()
     assert lib.fvmi_ensemble_substep(0, 0x1000, 0x2000, 0x3000, 0, 2, None, None) == -1 and "step" in _err()
+
+
+def test_field_stats_validation():
+    """The pitch must hold the interior and both ghost frames of the stated
+    widths (a one-ghost-cell field of width 126 has pitch 512 >= 128 * 4 but
+    not (126 + 4) * 4)."""
+    lib = _lib.lib()
+    f = ctypes.c_void_p(0x1000)
+    out = ctypes.c_void_p(0x2000)
+    assert lib.fvmi_field_stats(126, 8, 1, 1, f, 511, out, out, None) < 0
+    assert "field_stats" in _err()
+    assert lib.fvmi_field_stats(126, 8, -1, 1, f, 1024, out, out, None) < 0
+    assert lib.fvmi_field_stats(0, 8, 1, 1, f, 1024, out, out, None) < 0
+    assert lib.fvmi_field_stats(126, 8, 1, 1, None, 1024, out, out, None) < 0
