@@ -35,7 +35,7 @@ import numpy as np
 
 from . import _lib
 from .HipContext import Stream
-from .Simulator import BoundaryCondition, swe_initial_cfl
+from .Simulator import BoundaryCondition, resolve_math, swe_initial_cfl
 
 STRIP_COLS = 60
 
@@ -210,7 +210,7 @@ class Ensemble:
     ``instance_initial_cfl`` computes it, when already known).  Fields in
     page-locked memory upload without blocking the host."""
 
-    def __init__(self, context, scheme, instances=(), math="fast", steps_per_graph=8, waves_per_block=4,
+    def __init__(self, context, scheme, instances=(), math=None, steps_per_graph=8, waves_per_block=4,
                  seg_rows=None, min_seg_rows=24, waves_per_simd=4, use_graph=True, capacity=None, stream=None,
                  arena=None):
         self.logger = logging.getLogger(__name__)
@@ -220,7 +220,7 @@ class Ensemble:
         self.scheme = scheme
         self.scheme_id = _lib.SCHEME_IDS[scheme]
         self.nvars = 4 if scheme == "ee2d_kp07_dimsplit" else 3
-        self.math = math
+        self.math = math = resolve_math(scheme, math)  # None: the scheme's default policy
         self.launch = _lib.LaunchConfig(0, int(waves_per_block), _lib.FVMI_MATH_EXACT if math == "exact"
                                         else _lib.FVMI_MATH_FAST, 1)
         self.steps_per_graph = int(steps_per_graph)
@@ -518,6 +518,35 @@ class Ensemble:
         self._sync()
         return out
 
+    def nonfinite_counts(self, slots):
+        """Number of non-finite interior values of each instance in ``slots``
+        (every field of its state buffer), counted on the device; one copy
+        back for all of them."""
+        slots = [int(i) for i in slots]
+        if not slots:
+            return np.zeros(0, dtype=np.int64)
+        need = 16 * len(slots) * self.nvars
+        if getattr(self, "stats_cap", 0) < need:
+            if getattr(self, "stats_dev", None) is not None:
+                self._sync()
+                _lib.call("fvmi_free", self.stats_dev)
+            self.stats_dev = ctypes.c_void_p()
+            _lib.call("fvmi_malloc", ctypes.byref(self.stats_dev), need)
+            self.stats_cap = need
+        base = self.stats_dev.value
+        _lib.call("fvmi_fill_f32", base, need, need // 4, 1, 0.0, self.stream.handle)
+        for j, i in enumerate(slots):
+            I = self.host_table[i]
+            for k in range(self.nvars):
+                out = base + 16 * (j * self.nvars + k)
+                _lib.call("fvmi_field_stats", int(I.nx), int(I.ny), 2, 2, I.field[0][k], int(I.pitch), out, out + 8,
+                          self.stream.handle)
+        host = np.zeros(need // 8, dtype=np.float64)
+        _lib.call("fvmi_memcpy_async", host.ctypes.data, base, need, _lib.MEMCPY_DTOH, self.stream.handle)
+        self._sync()
+        bad = host.view(np.uint32)[2::4].reshape(len(slots), self.nvars)
+        return bad.sum(axis=1).astype(np.int64)
+
     def cell_steps(self):
         """Sum over live instances of nx*ny*steps taken."""
         self._download_table()
@@ -530,7 +559,7 @@ class Ensemble:
             self.graph = None
         if getattr(self, "stream", None) is not None:
             self.stream.synchronize()
-        for p in ("segs_dev", "nsegs_dev", "table", "count_dev"):
+        for p in ("segs_dev", "nsegs_dev", "table", "count_dev", "stats_dev"):
             ptr = getattr(self, p, None)
             if ptr is not None and ptr.value:
                 _lib.lib().fvmi_free(ptr)

@@ -32,7 +32,11 @@ from .HipContext import Stream
 from .Simulator import BoundaryCondition
 
 
-Completed = collections.namedtuple("Completed", "id scheme arrival admitted finished steps cells fields")
+# error: None for a request that ran to t_end with finite fields; otherwise
+# what went wrong ("non-finite ...": its state holds NaN or inf values when it
+# retires; "cancelled")
+Completed = collections.namedtuple("Completed", "id scheme arrival admitted finished steps cells fields error",
+                                   defaults=(None,))
 
 
 def request_spec(req, pinned=None):
@@ -102,7 +106,8 @@ class _Prefetcher:
             self.pool.shutdown(wait=True, cancel_futures=True)
         if self.pinned is not None:  # staged requests never admitted
             for f in self.futures.values():
-                if f.done() and f.exception() is None:
+                # a cancelled future has no result (f.exception() would raise)
+                if f.done() and not f.cancelled() and f.exception() is None:
                     for buf in f.result().get("_staging", ()):
                         self.pinned.put(buf)
         self.futures = {}
@@ -113,7 +118,7 @@ class Server:
     carving instance memory out of one DeviceArena, admission against
     ``budget_bytes`` of instance memory."""
 
-    def __init__(self, context, budget_bytes=200e9, steps_per_block=8, math="fast", capacity=4096,
+    def __init__(self, context, budget_bytes=200e9, steps_per_block=8, math=None, capacity=4096,
                  keep_results=True, use_graph=True, prefetch_workers=8, chunk_bytes=1 << 30):
         self.logger = logging.getLogger(__name__)
         self.context = context
@@ -132,6 +137,7 @@ class Server:
         self.owner = {}  # (scheme, slot) -> request record
         self.completed = []
         self.rejected = []
+        self.cancel_requested = set()  # ids; see cancel()
         self.timing = collections.Counter()
 
     def _pool(self, scheme):
@@ -157,19 +163,68 @@ class Server:
         self.owner[(req["scheme"], slot)] = {"req": req, "admitted": now}
         return slot
 
+    def cancel(self, request_id):
+        """Asks the server to drop a request: a waiting one is never admitted, a
+        running one leaves its pool at the next iteration (its slot and memory
+        are freed; the other instances of the pool are not affected).  Either
+        way it is reported with error "cancelled".  May be called from another
+        thread while serve() runs; a request may also carry ``cancel_at``
+        (seconds from the start of serve())."""
+        self.cancel_requested.add(request_id)
+
+    def _cancel(self, waiting, now_fn):
+        """Drops the requests asked for by cancel() or whose ``cancel_at`` has passed."""
+        now = now_fn()
+        ids = set(self.cancel_requested)
+        for rec in self.owner.values():
+            if rec["req"].get("cancel_at") is not None and rec["req"]["cancel_at"] <= now:
+                ids.add(rec["req"]["id"])
+        for req in waiting:
+            if req.get("cancel_at") is not None and req["cancel_at"] <= now:
+                ids.add(req["id"])
+        if not ids:
+            return 0
+        dropped = 0
+        for req in [r for r in waiting if r["id"] in ids]:
+            waiting.remove(req)
+            self.completed.append(Completed(req["id"], req["scheme"], req["arrival_time"], None, now, 0, 0, None,
+                                            "cancelled"))
+            self.cancel_requested.discard(req["id"])
+            dropped += 1
+        for (scheme, slot), rec in list(self.owner.items()):
+            req = rec["req"]
+            if req["id"] not in ids:
+                continue
+            pool = self.pools[scheme]
+            del self.owner[(scheme, slot)]
+            nt = int(pool.state(slot)[1])
+            self.used_bytes -= pool.bytes_of(req)
+            pool.remove(slot)  # waits for the pool's stream before the memory is reused
+            self.completed.append(Completed(req["id"], scheme, req["arrival_time"], rec["admitted"], now, nt,
+                                            int(req["nx"]) * int(req["ny"]) * nt, None, "cancelled"))
+            self.cancel_requested.discard(req["id"])
+            dropped += 1
+        return dropped
+
     def _retire(self, now_fn):
         done = 0
         for scheme, pool in self.pools.items():
             if not pool.live.any():
                 continue
-            for slot in pool.finished():  # downloads the pool's table once
+            slots = pool.finished()  # downloads the pool's table once
+            # failure isolation: an instance whose state went non-finite (NaN
+            # initial data, a blown-up solution) is reported as failed; the
+            # others of the pool never read its memory
+            bad = pool.nonfinite_counts(slots)
+            for slot, nbad in zip(slots, bad):
                 rec = self.owner.pop((scheme, slot))
                 req = rec["req"]
                 fields = pool.download(slot) if self.keep_results else None
                 nt = int(pool.tab["nt"][slot])
                 now = now_fn()
+                error = None if nbad == 0 else "non-finite values in %d cells of the final state" % int(nbad)
                 self.completed.append(Completed(req["id"], scheme, req["arrival_time"], rec["admitted"], now, nt,
-                                                int(req["nx"]) * int(req["ny"]) * nt, fields))
+                                                int(req["nx"]) * int(req["ny"]) * nt, fields, error))
                 self.used_bytes -= pool.bytes_of(req)
                 pool.remove(slot)
                 done += 1
@@ -216,6 +271,10 @@ class Server:
             while pending or waiting or self.owner:
                 while pending and (not realtime or pending[0]["arrival"] <= now()):
                     waiting.append(pending.popleft())
+                if self.cancel_requested or any("cancel_at" in r for r in requests):
+                    self._cancel(waiting, now)
+                    if not (pending or waiting or self.owner):
+                        break
                 prefetch.ahead(waiting)
                 c = clock()
                 busy = [pool for pool in self.pools.values() if pool.live.any()]
@@ -249,9 +308,12 @@ class Server:
         return self.summary(wall)
 
     def summary(self, wall):
-        lat = np.array([c.finished - c.arrival for c in self.completed])
+        ok = [c for c in self.completed if c.error is None]
+        lat = np.array([c.finished - c.arrival for c in ok])
         cells = sum(c.cells for c in self.completed)
-        out = {"requests": len(self.completed), "rejected": len(self.rejected), "wall_s": wall,
+        out = {"requests": len(ok), "rejected": len(self.rejected),
+               "failed": sum(1 for c in self.completed if c.error is not None and c.error != "cancelled"),
+               "cancelled": sum(1 for c in self.completed if c.error == "cancelled"), "wall_s": wall,
                "cell_updates_per_s": cells / wall if wall > 0 else 0.0,
                "p50_latency_s": float(np.percentile(lat, 50)) if lat.size else None,
                "p99_latency_s": float(np.percentile(lat, 99)) if lat.size else None,
@@ -269,7 +331,7 @@ class Server:
         self.pinned.close()
 
 
-def make_simulator(context, req, math="fast"):
+def make_simulator(context, req, math=None):
     """Plain Simulator for one request (the baseline path)."""
     from .EE2D_KP07_dimsplit import EE2D_KP07_dimsplit
     from .HLL2 import HLL2
@@ -286,7 +348,7 @@ def make_simulator(context, req, math="fast"):
     return cls(context, *f, req["nx"], req["ny"], req["dx"], req["dy"], req["g"], boundary_conditions=bc, math=math)
 
 
-def run_sequential(context, requests, realtime=True, math="fast", keep_results=True, prefetch_workers=8,
+def run_sequential(context, requests, realtime=True, math=None, keep_results=True, prefetch_workers=8,
                    progress=None):
     """Baseline: one plain simulate() per request, in arrival order (initial
     data materialised ahead on worker threads, as the server does)."""

@@ -116,3 +116,113 @@ def test_generated_requests_served_equal_sequential(ctx):
             assert np.array_equal(a, b), (c.id, c.scheme, k)
     assert summary["cell_steps"] == seq_summary["cell_steps"]
     server.close()
+
+
+def test_served_exact_requests_equal_the_oracle(ctx):
+    """One exact-arithmetic request of each scheme of the generated serving
+    workload, served together, equals the float32 oracle's simulate() of the
+    same request (the reference's dt policy), bitwise."""
+    from finitevolumegpu_amd.Ensemble import _bc_code
+    from finitevolumegpu_amd.Serving import Server, request_spec
+    from finitevolumegpu_amd.initial_conditions import serving_requests
+    from oracle import fv_oracle as O
+    reqs = serving_requests(40, seed=11, rate=1000.0, side_min=48, side_max=96, t_min=0.5, t_max=2.0)
+    picked, seen = [], set()
+    for r in reqs:
+        if r["scheme"] not in seen:
+            seen.add(r["scheme"])
+            picked.append(dict(r, arrival=0.0))
+    assert seen == {"kp07_dimsplit", "hll2", "waf", "ee2d_kp07_dimsplit"}
+    print("case served exact vs oracle", flush=True)
+    server = Server(ctx, steps_per_block=6, math="exact", capacity=8)
+    summary = server.serve([dict(r) for r in picked], realtime=False)
+    assert summary["requests"] == len(picked) and summary["failed"] == 0
+    by_id = {c.id: c for c in server.completed}
+    for r in picked:
+        spec = request_spec(r)
+        gamma = spec.get("gamma") if spec.get("gamma") is not None else None
+        ora = O.OracleSimulator(r["scheme"], spec["fields"], r["nx"], r["ny"], spec["dx"], spec["dy"], spec["g"],
+                                gamma=gamma, bc=_bc_code(spec["bc"]))
+        ora.simulate(spec["t_end"])
+        got = by_id[r["id"]]
+        assert got.steps == ora.nt, (r["scheme"], got.steps, ora.nt)
+        for k, (a, b) in enumerate(zip(got.fields, ora.download())):
+            assert np.array_equal(a, b), (r["scheme"], k, float(np.abs(a - b).max()))
+    server.close()
+
+
+def test_nan_request_is_reported_failed_and_isolated(ctx):
+    """A request whose initial data hold a NaN is reported as failed; the
+    requests served beside it equal their solo solves bitwise."""
+    from finitevolumegpu_amd.Serving import Server, make_simulator
+    reqs = _requests(count=8, seed=9)
+    bad_ids = []
+    for r in reqs:
+        if r["id"] in (1, 4):  # an HLL2 and a KP07_dimsplit request
+            f0 = np.array(r["fields"][0], dtype=np.float32)
+            f0[r["ny"] // 3, r["nx"] // 2] = np.nan if r["id"] == 1 else np.inf
+            r["fields"] = [f0] + list(r["fields"][1:])
+            bad_ids.append(r["id"])
+    print("case serving nan isolation", flush=True)
+    server = Server(ctx, steps_per_block=4, capacity=16)
+    summary = server.serve(reqs, realtime=False)
+    assert summary["failed"] == len(bad_ids)
+    assert summary["requests"] == len(reqs) - len(bad_ids)
+    by_id = {c.id: c for c in server.completed}
+    for r in reqs:
+        got = by_id[r["id"]]
+        if r["id"] in bad_ids:
+            assert got.error is not None and got.error.startswith("non-finite"), got.error
+            continue
+        assert got.error is None
+        sim = make_simulator(ctx, r)
+        sim.simulate(r["t_end"])
+        assert got.steps == sim.simSteps()
+        for k, (a, b) in enumerate(zip(got.fields, sim.download())):
+            assert np.array_equal(a, b), (r["id"], k)
+    assert server.used_bytes == 0
+    server.close()
+
+
+def test_cancelled_requests_leave_the_others_unchanged(ctx):
+    """A waiting request cancelled before admission is never run; a running
+    one cancelled mid-flight frees its slot and memory; both are reported as
+    cancelled, and the other requests equal their solo solves bitwise."""
+    from finitevolumegpu_amd.Serving import Server, make_simulator
+    reqs = _requests(count=6, seed=13)
+    long_req = dict(reqs[0])
+    long_req.update({"id": 50, "t_end": 1.0e4, "arrival": 0.0, "cancel_at": 0.05})  # would run ~10^5 steps
+    early = dict(reqs[1])
+    early.update({"id": 51, "arrival": 0.01, "cancel_at": 0.0})  # cancelled as soon as it arrives
+    print("case serving cancellation", flush=True)
+    server = Server(ctx, steps_per_block=4, capacity=16)
+    summary = server.serve(reqs + [long_req, early], realtime=True)
+    assert summary["cancelled"] == 2
+    by_id = {c.id: c for c in server.completed}
+    assert by_id[50].error == "cancelled" and by_id[50].admitted is not None and by_id[50].steps > 0
+    assert by_id[51].error == "cancelled" and by_id[51].admitted is None and by_id[51].steps == 0
+    for r in reqs:
+        got = by_id[r["id"]]
+        assert got.error is None
+        sim = make_simulator(ctx, r)
+        sim.simulate(r["t_end"])
+        for k, (a, b) in enumerate(zip(got.fields, sim.download())):
+            assert np.array_equal(a, b), (r["id"], k)
+    assert server.used_bytes == 0 and not server.owner
+    server.close()
+
+
+def test_cancel_from_another_thread(ctx):
+    """Server.cancel() from another thread stops a long request."""
+    import threading
+    from finitevolumegpu_amd.Serving import Server
+    req = _requests(count=1, seed=17)[0]
+    req.update({"t_end": 1.0e5})
+    server = Server(ctx, steps_per_block=4, capacity=4)
+    timer = threading.Timer(0.1, server.cancel, args=(req["id"],))
+    timer.start()
+    summary = server.serve([req], realtime=True, max_wall=30.0)
+    timer.join()
+    assert summary["cancelled"] == 1 and server.completed[0].error == "cancelled"
+    assert summary["wall_s"] < 10.0
+    server.close()

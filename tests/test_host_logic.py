@@ -156,3 +156,29 @@ def test_disk_generator_matches_direct_mask():
         want = np.where((x[None, :] - cx) ** 2 < r * r - (x[:, None] - cy) ** 2, 2.5, 1.0).astype(np.float32)
         got = _disk(n, cx, cy, r, 2.5)
         assert got.dtype == np.float32 and np.array_equal(got, want), (n, cx, cy, r)
+
+
+def test_prefetcher_close_skips_cancelled_futures():
+    """Closing the request prefetcher while materialisations are still queued
+    (serve() stopped early) cancels them without raising, and returns the
+    staging buffers of the ones that finished."""
+    import threading
+    from finitevolumegpu_amd.Serving import _Prefetcher
+
+    class FakePinned:
+        def __init__(self):
+            self.returned = []
+
+        def put(self, buf):
+            self.returned.append(buf)
+
+    pinned = FakePinned()
+    p = _Prefetcher(workers=1, pinned=pinned)
+    gate = threading.Event()
+    p.futures[0] = p.pool.submit(lambda: (gate.wait(5.0), {"_staging": ["buf0"]})[1])
+    for i in range(1, 4):  # queued behind the busy worker
+        p.futures[i] = p.pool.submit(lambda: {"_staging": ["never"]})
+    threading.Timer(0.05, gate.set).start()
+    p.close()
+    assert pinned.returned == ["buf0"]
+    assert p.futures == {}
