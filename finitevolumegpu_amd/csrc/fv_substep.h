@@ -800,8 +800,11 @@ __device__ __forceinline__ void setup_strip(const SubstepArgs& a, Strip& st, int
 // (p / xcds) / cus_per_xcd), and a SIMD's instruction arbiter favours its
 // oldest wave, so with equal segments a SIMD's waves finish one after another
 // and the last runs alone.  Here the waves of earlier rounds take longer parts
-// of each row group (weights kRoundWeights), and a CU's waves take
-// consecutive groups of one XCD's contiguous range of strips.
+// of each row group (weights kRoundWeights).  The waves of one workgroup take
+// the same rows of adjacent strips, so that each padded row is read as one
+// run of waves_per_block * 240 bytes at about the same time (DRAM pages are
+// opened once per row run instead of once per strip); consecutive CUs of an
+// XCD take consecutive row groups of the same strips.
 constexpr int kRounds = 4;
 constexpr int kRoundWeights[kRounds] = {13, 11, 9, 7};
 constexpr int round_weight_sum() {
@@ -832,10 +835,15 @@ __host__ __device__ inline bool round_segment_of(int block, int wave, int xcds, 
     const int k = q / cus_per_xcd;
     const int c = q - k * cus_per_xcd;
     if (k >= kRounds) return false;
-    const int G = (x * cus_per_xcd + c) * waves_per_block + wave;
-    if (G >= strips * gps) return false;
-    strip_x = G / gps;
-    const int g0 = (G - strip_x * gps) * grows;
+    // CU slot T takes row group T % gps of the strip band T / gps (a band is
+    // waves_per_block adjacent strips, one per wave)
+    const int T = x * cus_per_xcd + c;
+    const int bands = (strips + waves_per_block - 1) / waves_per_block;
+    if (T >= bands * gps) return false;
+    const int band = T / gps;
+    strip_x = band * waves_per_block + wave;
+    if (strip_x >= strips) return false;
+    const int g0 = (T - band * gps) * grows;
     const int n = min(grows, rows - g0);
     if (n <= 0) return false;
     int lo, hi;
@@ -851,7 +859,7 @@ __host__ __device__ inline bool round_split_fits(int strips, int rows, int waves
                                                  int& gps, int& grows) {
     // one 4-wave workgroup per CU and round: a wave on every SIMD
     if (waves_per_block != 4 || cu_count <= 0 || cu_count % 8 != 0 || strips <= 0) return false;
-    gps = (cu_count * waves_per_block) / strips;
+    gps = cu_count / ((strips + waves_per_block - 1) / waves_per_block);
     if (gps <= 0) return false;
     grows = (rows + gps - 1) / gps;
     const int last = rows - (gps - 1) * grows;  // the shortest (last) group
@@ -865,9 +873,12 @@ __host__ __device__ inline bool round_segment(const SubstepArgs& a, int block, i
 }
 
 // The segment of physical wave `wave` of workgroup `block` (of `grid`) in the
-// equal-segment plan.  Hardware dispatch places workgroup i on XCD i % 8; with
+// equal-segment plan: global wave gw takes segment gw % segs_per_strip of strip
+// gw / segs_per_strip.  Hardware dispatch places workgroup i on XCD i % 8; with
 // xcd_remap each XCD owns a contiguous range of logical blocks (neighbouring
-// strips share L2).
+// strips share L2).  (Giving a workgroup's waves adjacent strips instead, as
+// the round-weighted split does, made WAF 19 % slower: 174 -> 207 us per
+// 4096^2 substep.)
 __host__ __device__ inline bool uniform_segment(const SubstepArgs& a, int block, int grid, int wave, int& strip_x,
                                                 int& row_in, int& count) {
     int lb = block;
