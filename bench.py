@@ -49,7 +49,7 @@ def _parse():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=4096, help="grid side of the SWE workload")
-    ap.add_argument("--schemes", default="kp07,hll2,waf,ee2d,lxf,hll,force,kp07u")
+    ap.add_argument("--schemes", default="kp07,hll2,waf,ee2d,lxf,hll,force,kp07u,waf_fast,ee2d_fast")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--waves-per-block", type=int, default=0)
@@ -104,10 +104,10 @@ def _time_steps(sim, dt, steps, warmup, ctx):
     return end.time_since(start), wall
 
 
-def _swe_sim(ctx, scheme, n, args):
+def _swe_sim(ctx, scheme, n, args, math=None):
     from finitevolumegpu_amd.initial_conditions import dam_break_bumps
     h, hu, hv = dam_break_bumps(n, seed=0)
-    kw = {}
+    kw = {"math": math}
     if args.tile_rows:
         kw["tile_rows"] = args.tile_rows
     if args.waves_per_block:
@@ -140,7 +140,7 @@ STEP_BYTES = {"kp07": (48, 2), "hll2": (48, 2), "waf": (48, 2), "lxf": (24, 1), 
               "kp07u": (60, 2), "ee2d": (64, 2)}
 
 
-def _euler_sim(ctx, n):
+def _euler_sim(ctx, n, math=None):
     """One rank's 8192^2 subdomain: Rayleigh-Taylor, reflective north/south,
     periodic east/west through the one-rank MPISimulator (Dirichlet edges and
     an on-device halo exchange after every substep)."""
@@ -151,22 +151,27 @@ def _euler_sim(ctx, n):
     rho, ru, rv, E, dx, dy = rayleigh_taylor(n, n, seed=0)
     T = BoundaryCondition.Type
     bc = BoundaryCondition({"north": T.Reflective, "south": T.Reflective, "east": T.Periodic, "west": T.Periodic})
-    sim = MPISimulator(EE2D_KP07_dimsplit(ctx, rho, ru, rv, E, n, n, dx, dy, 0.1, 1.4, boundary_conditions=bc))
+    sim = MPISimulator(EE2D_KP07_dimsplit(ctx, rho, ru, rv, E, n, n, dx, dy, 0.1, 1.4, boundary_conditions=bc,
+                                          math=math))
     c = np.sqrt(1.4 * 2.6 / 1.0)
     dt = np.float32(0.45 * dx / c)
     return sim, dt
 
 
 def _measure(ctx, kind, args):
-    if kind == "ee2d":
+    """One scheme's timing; a kind ending in "_fast" runs that scheme in the
+    fast arithmetic policy instead of its default (Simulator.DEFAULT_MATH)."""
+    scheme = kind[:-5] if kind.endswith("_fast") else kind
+    math = "fast" if kind.endswith("_fast") else None
+    if scheme == "ee2d":
         n = 8192
-        sim, dt = _euler_sim(ctx, n)
+        sim, dt = _euler_sim(ctx, n, math)
         steps = max(1, args.steps // 3)
     else:
         n = args.n
-        sim, dt = _swe_sim(ctx, kind, n, args)
+        sim, dt = _swe_sim(ctx, scheme, n, args, math)
         steps = args.steps
-    step_bytes, per_step = STEP_BYTES[kind]
+    step_bytes, per_step = STEP_BYTES[scheme]
     ms, wall = _time_steps(sim, dt, steps, args.warmup, ctx)
     out = sim.download()
     finite = all(bool(np.isfinite(a).all()) for a in out)
@@ -177,7 +182,8 @@ def _measure(ctx, kind, args):
         "n": n, "steps": steps, "ms_per_step": ms / steps, "cell_updates_per_s": cells * steps / (ms * 1e-3),
         "avg_launch_ms": avg_launch_ms,
         "algorithmic_gbs": step_bytes * cells * steps / (ms * 1e-3) / 1e9,
-        "wall_s": wall, "finite": finite, "launch": [getattr(sim, "sim", sim).launch.tile_rows, getattr(sim, "sim", sim).launch.waves_per_block],
+        "wall_s": wall, "finite": finite, "math": getattr(sim, "sim", sim).math,
+        "launch": [getattr(sim, "sim", sim).launch.tile_rows, getattr(sim, "sim", sim).launch.waves_per_block],
     }
     del sim
     return res
@@ -223,7 +229,7 @@ def _ensemble_leg(ctx, count):
     # instance count, so a sample does not represent it
     sched = Autotuner.ensemble_schedule(ctx, "kp07_dimsplit", specs[::4], seg_rows=(16, 24, 32, 48), blocks=4)
     print("# ensemble schedule: %s" % json.dumps(sched), file=sys.stderr, flush=True)
-    ens = Ensemble(ctx, "kp07_dimsplit", specs, math="fast", steps_per_graph=8, seg_rows=sched["seg_rows"],
+    ens = Ensemble(ctx, "kp07_dimsplit", specs, steps_per_graph=8, seg_rows=sched["seg_rows"],
                    waves_per_block=sched["waves_per_block"])
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -236,8 +242,7 @@ def _ensemble_leg(ctx, count):
     wall_plain, cells_plain, checked, equal, steps_equal = 0.0, 0, 0, 0, 0
     for i, sp in enumerate(specs):
         bc = BoundaryCondition(BoundaryCondition.getTypes(_bc_code(sp["bc"])))
-        sim = KP07_dimsplit(ctx, *sp["fields"], sp["nx"], sp["ny"], 1.0, 1.0, 9.81, boundary_conditions=bc,
-                            math="fast")
+        sim = KP07_dimsplit(ctx, *sp["fields"], sp["nx"], sp["ny"], 1.0, 1.0, 9.81, boundary_conditions=bc)
         sim.synchronize()
         t0 = time.perf_counter()
         sim.simulate(sp["t_end"])
@@ -282,7 +287,7 @@ def _serving_leg(ctx, count, rate, sequential):
                       file=sys.stderr, flush=True)
         return report
 
-    server = Server(ctx, budget_bytes=200e9, steps_per_block=8, math="fast", keep_results=False)
+    server = Server(ctx, budget_bytes=200e9, steps_per_block=8, keep_results=False)
     served = server.serve([dict(r) for r in reqs], realtime=True, progress=progress("served"))
     server.close()
     print("# served: %s" % json.dumps(served), file=sys.stderr, flush=True)
@@ -291,7 +296,8 @@ def _serving_leg(ctx, count, rate, sequential):
                             progress=progress("sequential"))
     print("# sequential: %s" % json.dumps(seq), file=sys.stderr, flush=True)
     return {"workload": "%d requests, Poisson %.0f/s, 60%% KP07 / 20%% HLL2|WAF / 20%% EE2D, side log-U[128,2048], "
-                        "t_end U(2,40), budget 200 GB, fast arithmetic" % (count, rate),
+                        "t_end U(2,40), budget 200 GB, each scheme's default arithmetic (fast KP07_dimsplit and HLL2, "
+                        "exact WAF and EE2D)" % (count, rate),
             "served": served, "sequential": dict(seq, requests_of_total="%d/%d" % (k, count)),
             "throughput_ratio": served["cell_updates_per_s"] / seq["cell_updates_per_s"]}
 

@@ -158,7 +158,7 @@ class Autotuner:
 
     @staticmethod
     def ensemble_schedule(context, scheme, instances, seg_rows=(None, 24, 48, 96), waves_per_block=(2, 4),
-                          steps_per_graph=8, blocks=3, math="fast"):
+                          steps_per_graph=8, blocks=3, math=None):
         """Schedule of the batched engine for ``instances`` (Ensemble dicts): for
         every (segment rows, waves per workgroup) pair, builds the ensemble,
         replays one graph block to warm up and times ``blocks`` more with events.

@@ -33,7 +33,7 @@ class EE2D_KP07_dimsplit(DimsplitSimulator):
 
     def __init__(self, context, rho, rho_u, rho_v, E, nx, ny, dx, dy, g, gamma, theta=1.3, cfl_scale=0.9,
                  boundary_conditions=BoundaryCondition(), block_width=16, block_height=8,
-                 tile_rows=None, waves_per_block=None, math="fast", xcd_remap=True):
+                 tile_rows=None, waves_per_block=None, math=None, xcd_remap=True):
         super().__init__(context, nx, ny, dx, dy, boundary_conditions, cfl_scale, 2, block_width, block_height)
         self.g = np.float32(g)
         self.gamma = np.float32(gamma)

@@ -17,7 +17,7 @@ class FORCE(UnsplitSimulator):
 
     def __init__(self, context, h0, hu0, hv0, nx, ny, dx, dy, g, cfl_scale=0.9,
                  boundary_conditions=BoundaryCondition(), block_width=16, block_height=16,
-                 tile_rows=None, waves_per_block=None, math="fast", xcd_remap=True):
+                 tile_rows=None, waves_per_block=None, math=None, xcd_remap=True):
         super().__init__(context, nx, ny, dx, dy, boundary_conditions, cfl_scale, 1, block_width, block_height)
         self.g = np.float32(g)
         self._init_unsplit([h0, hu0, hv0], swe_initial_cfl(h0, hu0, hv0, self.dx, self.dy, g),

@@ -15,7 +15,7 @@ class HLL2(DimsplitSimulator):
 
     def __init__(self, context, h0, hu0, hv0, nx, ny, dx, dy, g, theta=1.8, cfl_scale=0.9,
                  boundary_conditions=BoundaryCondition(), block_width=16, block_height=16,
-                 tile_rows=None, waves_per_block=None, math="fast", xcd_remap=True):
+                 tile_rows=None, waves_per_block=None, math=None, xcd_remap=True):
         super().__init__(context, nx, ny, dx, dy, boundary_conditions, cfl_scale, 2, block_width, block_height)
         self.g = np.float32(g)
         self.theta = np.float32(theta)

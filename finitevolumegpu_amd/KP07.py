@@ -23,7 +23,7 @@ class KP07(UnsplitSimulator):
 
     def __init__(self, context, h0, hu0, hv0, nx, ny, dx, dy, g, theta=1.3, cfl_scale=0.9, order=2,
                  boundary_conditions=BoundaryCondition(), block_width=16, block_height=16,
-                 tile_rows=None, waves_per_block=None, math="fast", xcd_remap=True):
+                 tile_rows=None, waves_per_block=None, math=None, xcd_remap=True):
         if order not in (1, 2):
             raise ValueError("order must be 1 or 2")
         super().__init__(context, nx, ny, dx, dy, boundary_conditions, cfl_scale, order, block_width, block_height)

@@ -14,14 +14,15 @@ from .Simulator import BoundaryCondition, DimsplitSimulator, swe_initial_cfl
 class KP07_dimsplit(DimsplitSimulator):
     """Constructor signature of KP07_dimsplit.__init__ (KP07_dimsplit.py:55-64), plus
     the gfx950 launch shape (``tile_rows``, ``waves_per_block``), the arithmetic
-    policy ``math`` ('fast' | 'exact') and ``xcd_remap``."""
+    policy ``math`` ('fast' | 'exact' | None for the scheme's default,
+    Simulator.DEFAULT_MATH) and ``xcd_remap``."""
 
     scheme = "kp07_dimsplit"
     nvars = 3
 
     def __init__(self, context, h0, hu0, hv0, nx, ny, dx, dy, g, theta=1.3, cfl_scale=0.9,
                  boundary_conditions=BoundaryCondition(), block_width=16, block_height=16,
-                 tile_rows=None, waves_per_block=None, math="fast", xcd_remap=True):
+                 tile_rows=None, waves_per_block=None, math=None, xcd_remap=True):
         super().__init__(context, nx, ny, dx, dy, boundary_conditions, cfl_scale, 2, block_width, block_height)
         self.g = np.float32(g)
         self.theta = np.float32(theta)

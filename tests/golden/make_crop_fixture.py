@@ -1,0 +1,47 @@
+"""Writes tests/golden/crop512_waf.npz: the float32 oracle's WAF fields after
+500 full steps of the 512x512 version of the 4096x4096 throughput workload
+(the dam break with 16 Gaussian bumps, radius and widths scaled by 1/8,
+reflective walls, dt = 0.45 x the initial CFL), together with the float64
+oracle's distance from it (relative L1 per field).  The GPU test of the crop
+compares against this fixture instead of running the oracle, which takes
+minutes for WAF at this size.
+
+    python tests/golden/make_crop_fixture.py
+"""
+
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from finitevolumegpu_amd.initial_conditions import dam_break_bumps  # noqa: E402
+from oracle import fv_oracle as O  # noqa: E402
+
+N, STEPS, SCHEME = 512, 500, "waf"
+
+
+def main():
+    h, hu, hv = dam_break_bumps(N, seed=0)
+    t0 = time.time()
+    o32 = O.OracleSimulator(SCHEME, [h, hu, hv], N, N, 1.0, 1.0, 9.81, dtype=np.float32)
+    dt = np.float32(0.45 * float(o32.cfl))
+    o64 = O.OracleSimulator(SCHEME, [h, hu, hv], N, N, 1.0, 1.0, 9.81, dtype=np.float64)
+    for k in range(STEPS):
+        o32.step(dt)
+        o64.step(dt)
+        if (k + 1) % 50 == 0:
+            print("%d/%d steps, %.0f s" % (k + 1, STEPS, time.time() - t0), flush=True)
+    f32 = o32.download()
+    f64 = o64.download()
+    d64 = np.array([O.rel_l1(a, b) for a, b in zip(f32, f64)])
+    print("float32 vs float64 rel L1:", d64)
+    np.savez_compressed(os.path.join(HERE, "crop512_%s.npz" % SCHEME), h=f32[0], hu=f32[1], hv=f32[2],
+                        dt=np.float32(dt), steps=np.int32(STEPS), f32_vs_f64=d64, cfl=np.float32(o32.cfl))
+
+
+if __name__ == "__main__":
+    main()

@@ -8,9 +8,13 @@ both upwind branches, and where they meet head-on S_r <= 0 <= S_l (the
 reference tests S_l first).  Outflow edges keep the streams from wrapping
 round into a rarefaction that would dry the bed.
 
-* math='exact': bitwise equality of every field and of the CFL minimum.
-* math='fast' : relative L1 <= max(1e-5, 2 x the float32-vs-float64 oracle
-  difference) per field.
+* math='exact' (WAF's default): bitwise equality of every field and of the
+  CFL minimum.
+* math='fast' (the default of KP07_dimsplit, HLL2 and HLL): relative L1 <=
+  1e-5 per field.  WAF in fast arithmetic (not its default) is held to
+  max(1e-5, 2 x the float32-vs-float64 oracle difference): its limiter turns
+  rounding differences into O(1) changes of its weights, so no arithmetic
+  other than the exact one follows the float32 oracle to 1e-5 here.
 """
 
 import numpy as np
@@ -39,13 +43,16 @@ def supersonic_state(nx, ny, seed):
     return [np.ascontiguousarray(a, dtype=np.float32) for a in (h, h * u, h * v)]
 
 
-def _tolerance(got, ref, ref64):
+def _tolerance(got, ref, ref64, conditioned=False):
+    """Relative L1 <= 1e-5 per field; with ``conditioned`` (WAF in fast
+    arithmetic) <= max(1e-5, 2 x the float32 oracle's distance from float64)."""
     for k, (a, b) in enumerate(zip(got, ref)):
         err = O.rel_l1(a, b)
-        floor = O.rel_l1(b, ref64[k])
+        floor = O.rel_l1(b, ref64[k]) if ref64 is not None else 0.0
         print("field", k, "fast-vs-f32", err, "f32-vs-f64", floor, flush=True)
         assert np.all(np.isfinite(a)), k
-        assert err <= max(FAST_TOL, 2.0 * floor), (k, err, floor)
+        limit = max(FAST_TOL, 2.0 * floor) if conditioned else FAST_TOL
+        assert err <= limit, (k, err, floor)
 
 
 @pytest.mark.parametrize("scheme", ["kp07_dimsplit", "hll2", "waf"])
@@ -60,7 +67,7 @@ def test_dimsplit_supersonic(ctx, scheme, math):
     dt = np.float32(0.45 * float(ora.cfl))
     assert np.float32(sim.computeDt() * 2.0) == ora.cfl
     ora64 = None
-    if math == "fast":
+    if math == "fast" and scheme == "waf":
         ora64 = make_oracle(scheme, fields, NX, NY, 1.0, 1.0, BC, dtype=np.float64)
         for k, a in enumerate(padded):
             ora64.u0[k][:] = a
@@ -78,7 +85,7 @@ def test_dimsplit_supersonic(ctx, scheme, math):
         if scheme != "waf":
             assert np.float32(sim.computeDt() * 2.0) == ora.cfl
     else:
-        _tolerance(got, ref, ora64.download())
+        _tolerance(got, ref, ora64.download() if ora64 is not None else None, conditioned=scheme == "waf")
 
 
 @pytest.mark.parametrize("math", ["exact", "fast"])
@@ -86,8 +93,7 @@ def test_first_order_hll_supersonic(ctx, math):
     print("case hll", math, flush=True)
     fields = supersonic_state(NX, NY, 6)
     sim = _gpu(ctx, "hll", fields, NX, NY, BC, math)
-    oras = [U.OracleUnsplitSimulator("hll", fields, NX, NY, 1.0, 1.0, 9.81, bc=BC, dtype=dt)
-            for dt in ((np.float32,) if math == "exact" else (np.float32, np.float64))]
+    oras = [U.OracleUnsplitSimulator("hll", fields, NX, NY, 1.0, 1.0, 9.81, bc=BC, dtype=np.float32)]
     gc = U.SCHEME_GC["hll"]
     padded = _frame(fields, gc)
     for k, a in enumerate(padded):
@@ -108,4 +114,4 @@ def test_first_order_hll_supersonic(ctx, math):
         for k, (a, b) in enumerate(zip(got, ref)):
             assert np.array_equal(a, b), "field %d: max |diff| %g" % (k, np.abs(a - b).max())
     else:
-        _tolerance(got, ref, oras[1].download())
+        _tolerance(got, ref, None)
