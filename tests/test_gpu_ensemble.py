@@ -118,3 +118,53 @@ def test_autotuned_schedule_keeps_results(ctx):
         for k in range(3):
             assert np.array_equal(ens.download(i)[k], ref[k]), (i, k)
     ens.close()
+
+
+def test_generated_ensemble_workload(ctx):
+    """The batched-ensemble workload's own generator (64 dam breaks, side
+    log-uniform in [64, 1024], reflective or outflow walls, per-instance CFL dt),
+    with t_end shortened to U(1, 4): in the default (fast) arithmetic every
+    instance takes the plain simulate() loop's step count and every 8th equals
+    its plain solve bitwise; one instance in exact arithmetic equals the float32
+    oracle's simulate() bitwise."""
+    from finitevolumegpu_amd.Autotuner import Autotuner
+    from finitevolumegpu_amd.Ensemble import Ensemble, _bc_code
+    from finitevolumegpu_amd.initial_conditions import ensemble_instances
+    insts = ensemble_instances(64, seed=3, t_min=1.0, t_max=4.0)
+    sides = [d["nx"] for d in insts]
+    assert min(sides) < 100 and max(sides) > 600
+    specs = [{"fields": [d["h"], d["hu"], d["hv"]], "nx": d["nx"], "ny": d["ny"], "dx": 1.0, "dy": 1.0,
+              "g": 9.81, "t_end": d["t_end"], "bc": d["bc"]} for d in insts]
+    print("case generated ensemble", flush=True)
+    sched = Autotuner.ensemble_schedule(ctx, "kp07_dimsplit", specs[::8], seg_rows=(16, 32), waves_per_block=(2, 4),
+                                        blocks=1)
+    ens = Ensemble(ctx, "kp07_dimsplit", specs, steps_per_graph=8, seg_rows=sched["seg_rows"],
+                   waves_per_block=sched["waves_per_block"])
+    ens.run()
+    nt = ens.tab["nt"][:len(specs)].copy()
+    for i, spec in enumerate(specs):
+        if i % 8 == 0:
+            ref, t_ref, nt_ref = _plain(ctx, spec, None)
+            assert nt[i] == nt_ref
+            for k, (a, b) in enumerate(zip(ens.download(i), ref)):
+                assert np.array_equal(a, b), (i, k)
+        else:
+            from finitevolumegpu_amd.KP07_dimsplit import KP07_dimsplit
+            from finitevolumegpu_amd.Simulator import BoundaryCondition
+            bc = BoundaryCondition(BoundaryCondition.getTypes(_bc_code(spec["bc"])))
+            sim = KP07_dimsplit(ctx, *spec["fields"], spec["nx"], spec["ny"], 1.0, 1.0, 9.81, boundary_conditions=bc)
+            sim.simulate(spec["t_end"])
+            assert nt[i] == sim.simSteps(), i
+    ens.close()
+    # one instance (the smallest) in exact arithmetic against the oracle
+    j = int(np.argmin(sides))
+    spec = specs[j]
+    ens = Ensemble(ctx, "kp07_dimsplit", [spec], math="exact", steps_per_graph=8)
+    ens.run()
+    ora = O.OracleSimulator("kp07_dimsplit", spec["fields"], spec["nx"], spec["ny"], 1.0, 1.0, 9.81,
+                            bc=_bc_code(spec["bc"]))
+    ora.simulate(spec["t_end"])
+    assert int(ens.tab["nt"][0]) == ora.nt
+    for k, (a, b) in enumerate(zip(ens.download(0), ora.download())):
+        assert np.array_equal(a, b), k
+    ens.close()
