@@ -192,8 +192,8 @@ class ArakawaA2D:
         try:
             for i, v in enumerate(self.gpu_variables):
                 _lib.call("fvmi_fill_f32", scratch, 16, 4, 1, 0.0, _stream_handle(stream))
-                _lib.call("fvmi_field_stats", v.nx, v.ny, v.row_ptr(0), v.pitch, scratch.value,
-                          scratch.value + 8, _stream_handle(stream))
+                _lib.call("fvmi_field_stats", v.nx, v.ny, v.x_halo, v.y_halo, v.row_ptr(0), v.pitch,
+                          scratch.value, scratch.value + 8, _stream_handle(stream))
                 host = np.zeros(2, dtype=np.float64)
                 _lib.call("fvmi_memcpy_async", host.ctypes.data, scratch, 16, _lib.MEMCPY_DTOH, _stream_handle(stream))
                 if stream is not None:
@@ -372,131 +372,154 @@ def open_dataset(filename, *args, **kwargs):
     return netCDF4.Dataset(filename, *args, **kwargs)
 
 
+def unique_path(path):
+    """``path`` itself if no file has that name, else the first free name
+    ``<stem>_NNNN<ext>`` (NNNN = 0000, 0001, ...): output files are never
+    overwritten."""
+    if not os.path.isfile(path):
+        return path
+    stem, ext = os.path.splitext(path)
+    n = 0
+    while os.path.isfile("%s_%04d%s" % (stem, n, ext)):
+        n += 1
+    return "%s_%04d%s" % (stem, n, ext)
+
+
 class DataDumper:
-    """Opens and closes a netCDF dataset around a ``with`` block (Common.py:361-416).
+    """Context manager around one netCDF dataset (the role of the reference's
+    DataDumper, Common.py:361-416).  Creates the parent folder when needed; a
+    dataset opened for writing gets a name no existing file has (unique_path).
 
-        with DataDumper("out.nc", mode="w") as data:
-            data.ncfile.createDimension(...)
+        with DataDumper("out.nc", mode="w") as dump:
+            dump.ncfile.createDimension("x", 10)
 
-    In a writing mode an existing file is never overwritten: the name gets a
-    ``_0000``, ``_0001``, ... suffix.  Missing folders are created."""
+    ``filename`` is the path actually used; ``ncfile`` the open dataset."""
 
     def __init__(self, filename, *args, **kwargs):
         self.logger = logging.getLogger(__name__)
-        filename = os.path.abspath(filename)
-        dirname = os.path.dirname(filename)
-        if dirname and not os.path.isdir(dirname):
-            self.logger.info("Creating directory " + dirname)
-            os.makedirs(dirname)
-        mode = args[0] if args else kwargs.get("mode")
-        if mode and ("w" in mode or "+" in mode or "a" in mode):
-            stem, ext = os.path.splitext(filename)
-            i = 0
-            while os.path.isfile(filename):
-                filename = "{:s}_{:04d}{:s}".format(stem, i, ext)
-                i += 1
-        self.filename = os.path.abspath(filename)
-        self.args = args
-        self.kwargs = kwargs
-        self.logger.info("Initialized " + self.filename)
+        path = os.path.abspath(filename)
+        folder = os.path.dirname(path)
+        if folder and not os.path.isdir(folder):
+            os.makedirs(folder, exist_ok=True)
+            self.logger.info("made output folder %s", folder)
+        mode = kwargs.get("mode", args[0] if args else "r") or "r"
+        self.writing = any(c in mode for c in "wa+")
+        self.filename = unique_path(path) if self.writing else path
+        self.args, self.kwargs = args, kwargs
+        self.ncfile = None
 
     def __enter__(self):
-        self.logger.info("Opening " + self.filename)
         self.ncfile = open_dataset(self.filename, *self.args, **self.kwargs)
+        self.logger.info("%s opened (%s)", self.filename, "write" if self.writing else "read")
         return self
 
-    def __exit__(self, *args):
-        self.logger.info("Closing " + self.filename)
-        self.ncfile.close()
+    def __exit__(self, *exc):
+        if self.ncfile is not None:
+            self.ncfile.close()
+            self.ncfile = None
+            self.logger.info("%s closed", self.filename)
+        return False
 
 
-# state fields are left out of the sim_args attribute (too large)
+# Arguments holding initial fields: too large for the sim_args attribute.
 _FIELD_ARGS = ("rho", "rho_u", "rho_v", "E", "h0", "hu0", "hv0")
 
 
-def runSimulation(simulator, simulator_args, outfile, save_times, save_var_names=[], dt=None):
-    """Runs ``simulator(**simulator_args)`` and stores the variables named in
-    ``save_var_names`` (None entries are skipped, keeping the indices) at every
-    time in ``save_times`` in a netCDF file (Common.py:92-208): dimensions
-    (time, y, x), float32 coordinate variables (x and y from ``getExtent()``,
-    as the reference reads it), float32 ``(time, y, x)`` variables with zlib and
-    ``least_significant_digit=3``, and the global attributes created, git_hash,
-    git_status, simulator and sim_args (JSON).
+class _PhaseClock:
+    """Wall-clock sums per phase in the layout runSimulation returns:
+    {"start": {phase: sum of start times}, "end": {phase: sum of end times}},
+    so end - start is the total time spent in a phase."""
 
-    Returns ``(filename, profiling_data_sim_runner, profiling_data_mpi)``; the
-    last is the simulator's ``profiling_data_mpi`` (None for a simulator
-    without one, where the reference raises AttributeError).  The field
-    arguments of both the Euler and the shallow-water simulators are left out
-    of sim_args (the reference deletes the four Euler fields)."""
-    logger = logging.getLogger(__name__)
-    prof = {"start": {}, "end": {}}
-    for key in ("t_sim_init", "t_nc_write", "t_full_step"):
-        prof["start"][key] = 0
-        prof["end"][key] = 0
-    prof["start"]["t_sim_init"] = time.time()
-    save_times = np.asarray(save_times)
-    assert len(save_times) > 0, "Need to specify which times to save"
+    PHASES = ("t_sim_init", "t_nc_write", "t_full_step")
 
-    with Timer("construct") as t:
-        sim = simulator(**simulator_args)
-    logger.info("Constructed in " + str(t.secs) + " seconds")
+    def __init__(self):
+        self.data = {"start": dict.fromkeys(self.PHASES, 0), "end": dict.fromkeys(self.PHASES, 0)}
 
-    with DataDumper(outfile, mode="w", clobber=False) as outdata:
-        nc = outdata.ncfile
-        nc.created = time.ctime(time.time())
-        nc.git_hash = getGitHash()
-        nc.git_status = getGitStatus()
-        nc.simulator = str(simulator)
-        args_for_nc = {k: v for k, v in simulator_args.items() if k not in _FIELD_ARGS}
-        nc.sim_args = toJson(args_for_nc)
+    def start(self, phase):
+        self.data["start"][phase] += time.time()
 
+    def stop(self, phase):
+        self.data["end"][phase] += time.time()
+
+
+class _SnapshotFile:
+    """The netCDF layout of a simulation's output: dimensions (time, y, x),
+    float32 coordinate variables time, x, y, one float32 (time, y, x) variable
+    per saved field (zlib, least_significant_digit 3), and the global
+    attributes created, git_hash, git_status, simulator and sim_args (JSON).
+    This layout is the reference's (Common.py:117-163), so that its readers
+    open these files."""
+
+    def __init__(self, ncfile, simulator, simulator_args, sim, save_times, names):
+        ncfile.created = time.ctime(time.time())
+        ncfile.git_hash = getGitHash()
+        ncfile.git_status = getGitStatus()
+        ncfile.simulator = str(simulator)
+        ncfile.sim_args = toJson({k: v for k, v in simulator_args.items() if k not in _FIELD_ARGS})
         nx, ny = int(simulator_args["nx"]), int(simulator_args["ny"])
-        nc.createDimension("time", len(save_times))
-        nc.createDimension("x", nx)
-        nc.createDimension("y", ny)
-        ncvars = {
-            "time": nc.createVariable("time", np.dtype("float32").char, "time"),
-            "x": nc.createVariable("x", np.dtype("float32").char, "x"),
-            "y": nc.createVariable("y", np.dtype("float32").char, "y"),
-        }
-        ncvars["time"][:] = save_times
-        extent = sim.getExtent()
-        ncvars["x"][:] = np.linspace(extent[0], extent[1], nx)
-        ncvars["y"][:] = np.linspace(extent[2], extent[3], ny)
+        for name, size in (("time", len(save_times)), ("x", nx), ("y", ny)):
+            ncfile.createDimension(name, size)
+        f32 = np.dtype(np.float32).char
+        x0, x1, y0, y1 = sim.getExtent()
+        for name, values in (("time", save_times), ("x", np.linspace(x0, x1, nx)), ("y", np.linspace(y0, y1, ny))):
+            ncfile.createVariable(name, f32, name)[:] = values
+        self.fields = [ncfile.createVariable(name, f32, ("time", "y", "x"), zlib=True, least_significant_digit=3)
+                       for name in names]
 
-        download_vars = [i for i, name in enumerate(save_var_names) if name is not None]
-        names = [save_var_names[i] for i in download_vars]
-        for name in names:
-            ncvars[name] = nc.createVariable(name, np.dtype("float32").char, ("time", "y", "x"), zlib=True,
-                                             least_significant_digit=3)
+    def write(self, index, values):
+        for var, value in zip(self.fields, values):
+            var[index, :] = value
 
-        t_steps = np.empty_like(save_times)
-        t_steps[0] = save_times[0]
-        t_steps[1:] = save_times[1:] - save_times[:-1]
-        prof["end"]["t_sim_init"] = time.time()
 
+def runSimulation(simulator, simulator_args, outfile, save_times, save_var_names=[], dt=None):
+    """Constructs ``simulator(**simulator_args)``, advances it to each of
+    ``save_times`` in turn and stores the variables named in
+    ``save_var_names`` (an entry None skips the variable of that index) at
+    each of them, in a new netCDF file named after ``outfile`` (the layout of
+    _SnapshotFile).  ``dt``: a fixed step for simulate(), None for its CFL
+    policy.  The simulator's check() runs before every interval; a failing
+    check ends the run early, logged, with the frames written so far.
+
+    Returns ``(filename, phase_times, profiling_data_mpi)``: the file written,
+    the wall-clock sums of _PhaseClock, and the simulator's
+    ``profiling_data_mpi`` (None for a simulator without one).  Signature and
+    return tuple of the reference's runSimulation (Common.py:92-208)."""
+    logger = logging.getLogger(__name__)
+    save_times = np.asarray(save_times)
+    if save_times.size == 0:  # the reference asserts here
+        raise AssertionError("runSimulation: save_times is empty")
+    clock = _PhaseClock()
+    clock.start("t_sim_init")
+    t0 = time.time()
+    sim = simulator(**simulator_args)
+    logger.info("%s constructed in %.3f s", simulator.__name__ if hasattr(simulator, "__name__") else simulator,
+                time.time() - t0)
+    keep = [(i, name) for i, name in enumerate(save_var_names) if name is not None]
+    # interval k advances from save_times[k-1] (0 for the first) to save_times[k]
+    intervals = np.diff(save_times, prepend=0)
+
+    with DataDumper(outfile, mode="w", clobber=False) as dump:
+        snaps = _SnapshotFile(dump.ncfile, simulator, simulator_args, sim, save_times, [n for _, n in keep])
+        clock.stop("t_sim_init")
         printer = ProgressPrinter(save_times[-1], print_every=10)
-        t_end = 0.0
-        for k in range(len(save_times)):
-            t_step, t_end = t_steps[k], save_times[k]
+        for k, interval in enumerate(intervals):
             try:
                 sim.check()
             except AssertionError as e:
-                logger.error("Error after %d steps (t=%f: %s", sim.simSteps(), sim.simTime(), str(e))
-                return outdata.filename, prof, getattr(sim, "profiling_data_mpi", None)
-            prof["start"]["t_full_step"] += time.time()
-            if t_step > 0.0:
-                sim.simulate(t_step, dt)
-            prof["end"]["t_full_step"] += time.time()
-            prof["start"]["t_nc_write"] += time.time()
-            values = sim.download(download_vars)
-            for i, name in enumerate(names):
-                ncvars[name][k, :] = values[i]
-            prof["end"]["t_nc_write"] += time.time()
-            line = printer.getPrintString(t_end)
-            if line:
-                logger.debug(line)
+                logger.error("stopped before interval %d: check failed at step %d, t=%g (%s)", k, sim.simSteps(),
+                             sim.simTime(), e)
+                break
+            clock.start("t_full_step")
+            if interval > 0.0:
+                sim.simulate(interval, dt)
+            clock.stop("t_full_step")
+            clock.start("t_nc_write")
+            snaps.write(k, sim.download([i for i, _ in keep]))
+            clock.stop("t_nc_write")
+            progress = printer.getPrintString(save_times[k])
+            if progress:
+                logger.debug(progress)
         if sim.simSteps() > 0:
-            logger.debug("Simulated to t=%f in %d timesteps (average dt=%f)", t_end, sim.simSteps(),
+            logger.debug("%d steps to t=%g, mean dt %g", sim.simSteps(), sim.simTime(),
                          sim.simTime() / sim.simSteps())
-    return outdata.filename, prof, getattr(sim, "profiling_data_mpi", None)
+    return dump.filename, clock.data, getattr(sim, "profiling_data_mpi", None)
