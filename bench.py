@@ -104,6 +104,20 @@ def _time_steps(sim, dt, steps, warmup, ctx):
     return end.time_since(start), wall
 
 
+def _steady_ms(sim, dt, steps):
+    """Median time of one step over ``steps`` further steps, each between two
+    events: the rate once the shader clock has settled (reported beside the
+    timed region, never as ``value``)."""
+    from finitevolumegpu_amd.HipContext import Event
+    evs = [Event() for _ in range(steps + 1)]
+    evs[0].record(sim.stream)
+    for i in range(steps):
+        sim.step(dt)
+        evs[i + 1].record(sim.stream)
+    evs[-1].synchronize()
+    return float(np.median([evs[i + 1].time_since(evs[i]) for i in range(steps)]))
+
+
 def _swe_sim(ctx, scheme, n, args, math=None):
     from finitevolumegpu_amd.initial_conditions import dam_break_bumps
     h, hu, hv = dam_break_bumps(n, seed=0)
@@ -173,6 +187,7 @@ def _measure(ctx, kind, args):
         steps = args.steps
     step_bytes, per_step = STEP_BYTES[scheme]
     ms, wall = _time_steps(sim, dt, steps, args.warmup, ctx)
+    steady = _steady_ms(sim, dt, 30 if scheme == "ee2d" else 100)
     out = sim.download()
     finite = all(bool(np.isfinite(a).all()) for a in out)
     launches = per_step * steps
@@ -183,6 +198,7 @@ def _measure(ctx, kind, args):
         "avg_launch_ms": avg_launch_ms,
         "algorithmic_gbs": step_bytes * cells * steps / (ms * 1e-3) / 1e9,
         "wall_s": wall, "finite": finite, "math": getattr(sim, "sim", sim).math,
+        "steady_ms_per_step": steady,
         "launch": [getattr(sim, "sim", sim).launch.tile_rows, getattr(sim, "sim", sim).launch.waves_per_block],
     }
     del sim

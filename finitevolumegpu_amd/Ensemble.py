@@ -37,7 +37,11 @@ from . import _lib
 from .HipContext import Stream
 from .Simulator import BoundaryCondition, resolve_math, swe_initial_cfl
 
-STRIP_COLS = 60
+def strip_cols(scheme, math):
+    """Output columns of one strip of a scheme's march (strip_cols<kC> in
+    fv_substep.h): 124 where it marches two columns per lane
+    (EE2D_KP07_dimsplit in fast arithmetic), else 60."""
+    return 124 if (scheme == "ee2d_kp07_dimsplit" and math == "fast") else 60
 
 
 def _bc_code(bc):
@@ -53,13 +57,13 @@ def _bc_code(bc):
     return int(bc)
 
 
-def plan_segments(ids, nx, ny, seg):
+def plan_segments(ids, nx, ny, seg, strip_cols=124):
     """Work list of the instances ``ids`` (sizes ``nx``, ``ny``): one int4
     (instance, strip, first row, rows) per run of at most ``seg`` rows of a
-    60-column strip, every strip of every instance covered exactly once,
-    longest segments first (ties in instance, strip, row order)."""
+    ``strip_cols``-column strip, every strip of every instance covered exactly
+    once, longest segments first (ties in instance, strip, row order)."""
     ids = np.asarray(ids, dtype=np.int64)
-    sx = (np.asarray(nx, dtype=np.int64) + STRIP_COLS - 1) // STRIP_COLS
+    sx = (np.asarray(nx, dtype=np.int64) + strip_cols - 1) // strip_cols
     ny = np.asarray(ny, dtype=np.int64)
     per_strip = (ny + seg - 1) // seg
     count = sx * per_strip
@@ -338,10 +342,11 @@ class Ensemble:
         return self._running()
 
     def _strip_rows(self, mask):
-        """Rows of 60-column strips of the instances in ``mask`` (sum over them)."""
+        """Rows of strips of the instances in ``mask`` (sum over them)."""
         nx = self.tab["nx"][mask].astype(np.int64)
         ny = self.tab["ny"][mask].astype(np.int64)
-        return int(((nx + STRIP_COLS - 1) // STRIP_COLS * ny).sum())
+        w = strip_cols(self.scheme, self.math)
+        return int(((nx + w - 1) // w * ny).sum())
 
     # ------------------------------------------------------------------
     # slots
@@ -391,7 +396,7 @@ class Ensemble:
 
     def plan(self, active):
         """Work list for the instances in ``active``: segments of at most seg_rows
-        rows, never crossing a 60-column strip, longest first.  The list and its
+        rows, never crossing a strip, longest first.  The list and its
         length are written to the device on the ensemble's stream; the graph
         is captured again only if the list outgrows its buffer."""
         active = np.asarray(active, dtype=bool) & self.live
@@ -401,7 +406,8 @@ class Ensemble:
         else:
             slots = self.context.cu_count * 4 * self.waves_per_simd
             seg = max(self.min_seg_rows, -(-total // max(1, slots)))
-        arr = plan_segments(np.flatnonzero(active), self.tab["nx"][active], self.tab["ny"][active], seg)
+        arr = plan_segments(np.flatnonzero(active), self.tab["nx"][active], self.tab["ny"][active], seg,
+                            strip_cols(self.scheme, self.math))
         need = len(arr)
         self._sync()  # the pinned staging buffer may still feed the previous upload
         if self.segs_dev is None or need > self.segs_cap:

@@ -30,7 +30,7 @@ namespace fvmi {
 
 struct EnsembleArgs {
     fvmi_instance_t* table;
-    const int4* segs;     // (instance, strip_x, row_in, count)
+    const int4* segs;     // (instance, strip_x, row_in, count); strips of strip_cols<S::kC>() columns
     const int* num_segs;  // device: valid entries of segs
     int step;
     int waves_per_block;
@@ -96,15 +96,10 @@ __device__ __forceinline__ void ensemble_segment(const EnsembleArgs& e, const in
     a.dtdy = a.dt / a.dy;
     set_slope_constants(a);
 
-    Strip st;
+    Strip<S::kC> st;
     setup_strip<S, M>(a, st, lane, __builtin_amdgcn_readfirstlane(seg.y), __builtin_amdgcn_readfirstlane(seg.z),
                       __builtin_amdgcn_readfirstlane(seg.w));
-    float acc;
-    if constexpr (S::kSlopes) {
-        acc = (e.step == 0) ? march_slopes<S, M, true>(a, st) : march_slopes<S, M, false>(a, st);
-    } else {
-        acc = (e.step == 0) ? march_waf<S, M, true>(a, st) : march_waf<S, M, false>(a, st);
-    }
+    const float acc = march<S, M>(a, st);
     if (e.step != 1) return;  // the CFL is taken after the y-then-x substep
     float dt_min = acc;
     if constexpr (!M::exact) dt_min = (acc > 0.0f) ? fast::rcp(acc) : __builtin_inff();

@@ -10,6 +10,22 @@ namespace fvmi {
 
 enum SchemeId : int { SCHEME_KP07 = 0, SCHEME_HLL2 = 1, SCHEME_WAF = 2, SCHEME_EULER = 3 };
 
+// Columns per lane of each scheme's march in each arithmetic policy (the kC
+// of its policy struct) and the output columns of one of its strips.
+inline int scheme_cols_per_lane(int scheme, int math) {
+    static_assert(SweKP07<MathFast>::kC == FVMI_SWE_COLS && SweHLL2<MathFast>::kC == FVMI_SWE_COLS &&
+                      SweWAF<MathFast>::kC == 1 && EulerKP07<MathFast>::kC == 2 &&
+                      SweKP07<MathExact>::kC == FVMI_SWE_COLS && SweHLL2<MathExact>::kC == FVMI_SWE_COLS &&
+                      SweWAF<MathExact>::kC == 1 && EulerKP07<MathExact>::kC == 1,
+                  "scheme_cols_per_lane follows the policy structs");
+    if (scheme == SCHEME_WAF) return 1;
+    if (scheme == SCHEME_EULER) return math == FVMI_MATH_EXACT ? 1 : 2;
+    return FVMI_SWE_COLS;
+}
+inline int scheme_strip_cols(int scheme, int math) {
+    return scheme_cols_per_lane(scheme, math) == 2 ? strip_cols<2>() : strip_cols<1>();
+}
+
 // Fills the decomposition fields of `a` (sub-area, launch shape, device size)
 // and returns the grid size to launch.  seg_rows == 0 picks the shape:
 //   * the round-weighted plan (round_segment) when `round_ok`, the workgroups
@@ -23,7 +39,7 @@ inline int plan_substep(SubstepArgs& a, int cu_count, bool round_ok = false, int
                         int min_seg = 24, int min_part = 16) {
     const int wx = a.x1 - a.x0;
     a.rows_per_strip = a.y1 - a.y0;
-    a.strips_x = (wx + kStripCols - 1) / kStripCols;
+    a.strips_x = (wx + a.strip_cols - 1) / a.strip_cols;
     a.round_plan = 0;
     a.xcds = 8;
     a.cus_per_xcd = cu_count / 8;

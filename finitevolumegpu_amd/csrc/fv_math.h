@@ -21,11 +21,45 @@
 
 namespace fvmi {
 
+// Correctly rounded float32 division and square root in fewer instructions
+// than the compiler's general IEEE sequences, for operands in the normal range
+// (no scaling for denormal or overflowing intermediates, no special-case
+// fix-up: a zero or infinite divisor gives NaN).
+//  * reciprocal: v_rcp_f32 (1 ulp) refined by one Newton step with FMA,
+//    y = y0 + y0 (1 - b y0), which rounds to the correctly rounded 1/b;
+//  * quotient: q0 = a y, the exact remainder r = a - b q0 by FMA, and
+//    q = q0 + r y: correctly rounded when y is (Markstein's theorem); a zero
+//    quotient is q0 itself, whose sign is that of a b (q0 + r y would turn
+//    -0 into +0, and the sign of a zero reaches copysign in the WAF limiter);
+//  * square root: v_sqrt_f32 (1 ulp) and its two neighbours, the one whose
+//    residual x - s s' changes sign chosen by two FMAs.
+// Checked against the IEEE sequences on every mantissa of [1, 2) (reciprocal)
+// and [1, 4) (square root) and on random quotients, and in every exact-
+// arithmetic test, which compare the fields bitwise with the float32 oracle.
+__device__ __forceinline__ float rcp_rn(float b) {
+    const float y = __builtin_amdgcn_rcpf(b);
+    return __builtin_fmaf(__builtin_fmaf(-b, y, 1.0f), y, y);
+}
+__device__ __forceinline__ float div_rn(float a, float b) {
+    const float y = rcp_rn(b);
+    const float q = a * y;
+    const float qc = __builtin_fmaf(__builtin_fmaf(-b, q, a), y, q);
+    return q == 0.0f ? q : qc;
+}
+__device__ __forceinline__ float sqrt_rn(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float dn = __uint_as_float(__float_as_uint(s) - 1u);
+    const float up = __uint_as_float(__float_as_uint(s) + 1u);
+    float r = (__builtin_fmaf(-dn, s, x) <= 0.0f) ? dn : s;
+    r = (__builtin_fmaf(-up, s, x) > 0.0f) ? up : r;
+    return r;
+}
+
 struct MathExact {
     static constexpr bool exact = true;
-    __device__ static __forceinline__ float div(float a, float b) { return a / b; }
-    __device__ static __forceinline__ float sqrt(float x) { return __builtin_sqrtf(x); }
-    __device__ static __forceinline__ float rcp(float x) { return 1.0f / x; }
+    __device__ static __forceinline__ float div(float a, float b) { return div_rn(a, b); }
+    __device__ static __forceinline__ float sqrt(float x) { return sqrt_rn(x); }
+    __device__ static __forceinline__ float rcp(float x) { return rcp_rn(x); }
 };
 
 struct MathFast {
@@ -82,8 +116,8 @@ template <class M>
 __device__ __forceinline__ void swe_F(const float q[3], float g, float F[3]) {
     if constexpr (M::exact) {
         F[0] = q[1];
-        F[1] = q[1] * q[1] / q[0] + 0.5f * g * q[0] * q[0];
-        F[2] = q[1] * q[2] / q[0];
+        F[1] = M::div(q[1] * q[1], q[0]) + 0.5f * g * q[0] * q[0];
+        F[2] = M::div(q[1] * q[2], q[0]);
     } else {
         const float u = q[1] * M::rcp(q[0]);
         F[0] = q[1];
@@ -98,17 +132,17 @@ __device__ __forceinline__ void swe_cu_flux(const float qm[3], const float qp[3]
     if constexpr (M::exact) {
         float Fp[3], Fm[3];
         swe_F<M>(qp, g, Fp);
-        const float up = qp[1] / qp[0];
+        const float up = M::div(qp[1], qp[0]);
         const float cp = M::sqrt(g * qp[0]);
         swe_F<M>(qm, g, Fm);
-        const float um = qm[1] / qm[0];
+        const float um = M::div(qm[1], qm[0]);
         const float cm = M::sqrt(g * qm[0]);
         const float am = fminf(fminf(um - cm, up - cp), 0.0f);
         const float ap = fmaxf(fmaxf(um + cm, up + cp), 0.0f);
         const float apam = ap * am;
         const float den = ap - am;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) out[k] = ((ap * Fm[k] - am * Fp[k]) + apam * (qp[k] - qm[k])) / den;
+        for (int k = 0; k < 3; ++k) out[k] = M::div((ap * Fm[k] - am * Fp[k]) + apam * (qp[k] - qm[k]), den);
     } else {
         const float ip = M::rcp(qp[0]);
         const float im = M::rcp(qm[0]);
@@ -132,8 +166,8 @@ __device__ __forceinline__ void swe_hll_flux(const float ql[3], const float qr[3
     const float h_l = ql[0], h_r = qr[0];
     float u_l, u_r;
     if constexpr (M::exact) {
-        u_l = ql[1] / h_l;
-        u_r = qr[1] / h_r;
+        u_l = M::div(ql[1], h_l);
+        u_r = M::div(qr[1], h_r);
     } else {
         u_l = ql[1] * M::rcp(h_l);
         u_r = qr[1] * M::rcp(h_r);
@@ -155,7 +189,7 @@ __device__ __forceinline__ void swe_hll_flux(const float ql[3], const float qr[3
         const float den = S_r - S_l;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const float star = (S_r * F_l[k] - S_l * F_r[k] + srsl * (qr[k] - ql[k])) / den;
+            const float star = M::div(S_r * F_l[k] - S_l * F_r[k] + srsl * (qr[k] - ql[k]), den);
             out[k] = (S_l >= 0.0f) ? F_l[k] : ((S_r <= 0.0f) ? F_r[k] : star);
         }
     } else {
@@ -199,8 +233,8 @@ __device__ __forceinline__ WafCell waf_cell(const float q[3], float g, float sqr
     p.hu = q[1];
     p.hv = q[2];
     if constexpr (M::exact) {
-        p.u = q[1] / q[0];
-        p.v = q[2] / q[0];
+        p.u = M::div(q[1], q[0]);
+        p.v = M::div(q[2], q[0]);
         p.c = M::sqrt(g * q[0]);
         p.ih = 0.0f;
     } else {
@@ -297,9 +331,9 @@ __device__ __forceinline__ void waf_flux_core(const WafCell& L, const WafCell& R
             F_2[k] = F_1[k] + S_l * (Q_star_l[k] - q_l1[k]);
             F_3[k] = F_4[k] + S_r * (Q_star_r[k] - q_r1[k]);
         }
-        const float c_1 = S_l * dt / d;
-        const float c_2 = S_star * dt / d;
-        const float c_3 = S_r * dt / d;
+        const float c_1 = M::div(S_l * dt, d);
+        const float c_2 = M::div(S_star * dt, d);
+        const float c_3 = M::div(S_r * dt, d);
         // desingularize((c_1 > 0) ? h_dag_l - h_l2 : h_dag_r - h_r) / desingularize(h_dag - h_l), ...
         const float r_1 = M::div((c_1 > 0.0f) ? Dl.d1 : Dr.d1, D.d1);
         const float r_2 = M::div((c_2 > 0.0f) ? Dl.d2 : Dr.d2, D.d2);
@@ -356,10 +390,10 @@ template <class M>
 __device__ __forceinline__ float swe_cell_dt(const float q[3], float dx, float dy, float g) {
     if constexpr (M::exact) {
         const float h = q[0];
-        const float u = q[1] / h;
-        const float v = q[2] / h;
-        const float max_u = dx / (fabsf(u) + M::sqrt(g * h));
-        const float max_v = dy / (fabsf(v) + M::sqrt(g * h));
+        const float u = M::div(q[1], h);
+        const float v = M::div(q[2], h);
+        const float max_u = M::div(dx, fabsf(u) + M::sqrt(g * h));
+        const float max_v = M::div(dy, fabsf(v) + M::sqrt(g * h));
         return fminf(max_u, max_v);
     } else {
         const float ih = M::rcp(q[0]);
@@ -413,7 +447,7 @@ __device__ __forceinline__ void euler_hll_flux(const float ql[4], const float qr
         const float den = S_r - S_l;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const float star = (S_r * F_l[k] - S_l * F_r[k] + srsl * (qr[k] - ql[k])) / den;
+            const float star = M::div(S_r * F_l[k] - S_l * F_r[k] + srsl * (qr[k] - ql[k]), den);
             out[k] = (S_l >= 0.0f) ? F_l[k] : ((S_r <= 0.0f) ? F_r[k] : star);
         }
     } else {
@@ -444,7 +478,7 @@ __device__ __forceinline__ void euler_cu_flux(const float qm[4], const float qp[
     if constexpr (M::exact) {
         const float den = ap - am;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) out[k] = ((ap * Fm[k] - am * Fp[k]) + apam * (qp[k] - qm[k])) / den;
+        for (int k = 0; k < 4; ++k) out[k] = M::div((ap * Fm[k] - am * Fp[k]) + apam * (qp[k] - qm[k]), den);
     } else {
         const float iden = M::rcp(ap - am);
 #pragma unroll

@@ -6,10 +6,13 @@
 // Euler KP07DimsplitKernel (EE2D_KP07_dimsplit.cu:126-247).
 //
 // Work decomposition (gfx950, 64-lane wavefronts).  A wavefront owns a
-// "strip": 60 output columns x `tile_rows` output rows.  Lane c holds padded
-// column x0+c (x0 = 60*strip), so lanes 2..61 are the strip's own columns and
-// lanes 0,1,62,63 its x halo.  The wave marches down the strip one padded row
-// at a time (tile_rows+4 rows including the 2+2 y halo):
+// "strip" of output columns x `tile_rows` output rows, C padded columns per
+// lane: with C = 1 lane c holds padded column x0+c (x0 = 60*strip), lanes
+// 2..61 are the strip's own 60 columns and lanes 0,1,62,63 its x halo; with
+// C = 2 lane c holds columns x0+2c, x0+2c+1 (x0 = 124*strip), lanes 1..62 the
+// strip's own 124 columns and lanes 0, 63 the halo, each row read and written
+// with 8-byte accesses.  The wave marches down the strip one padded row at a
+// time (tile_rows+4 rows including the 2+2 y halo):
 //   * the x sweep of a row is done across lanes, neighbours exchanged with
 //     cross-lane moves, no LDS and no barrier;
 //   * the y sweep is done per lane with a register window of the last 3
@@ -36,7 +39,19 @@ namespace fvmi {
 enum : int { BC_DIRICHLET = 0, BC_NEUMANN = 1, BC_PERIODIC = 2, BC_REFLECTIVE = 3, BC_OUTFLOW = 4 };
 
 constexpr int kLanes = 64;
-constexpr int kStripCols = 60;  // output columns per wavefront
+
+// A wavefront marches a strip of the domain, C padded columns per lane.  The
+// lanes at each end that hold the strip's 2-column x halo are not written:
+// halo_lanes<C>() of them, leaving strip_cols<C>() output columns (60 for one
+// column per lane, 124 for two).
+template <int C>
+__host__ __device__ constexpr int halo_lanes() {
+    return (2 + C - 1) / C;
+}
+template <int C>
+__host__ __device__ constexpr int strip_cols() {
+    return (kLanes - 2 * halo_lanes<C>()) * C;
+}
 
 struct SubstepArgs {
     int nx, ny;
@@ -49,13 +64,14 @@ struct SubstepArgs {
     float* out[4];
     int out_pitch[4];
     float* cfl;  // two-slot CFL buffer: a step-1 launch minimises slot 1, every launch resets slot 1-step; may be null
-    // Work decomposition: the sub-area is cut into 60-column strips and every
+    // Work decomposition: the sub-area is cut into strips of strip_cols columns and every
     // strip into `segs_per_strip` segments of `seg_rows` rows (the last one
     // shorter), one per wavefront.  Segment j starts at the same row in every
     // strip, so the waves of neighbouring strips (on the same XCD) read the
     // cache lines they share at about the same time.
     int seg_rows;
     int segs_per_strip;
+    int strip_cols;  // output columns per strip (strip_cols<S::kC>() of the scheme launched)
     int strips_x;
     int rows_per_strip;  // y1 - y0
     int num_waves;       // strips_x * segs_per_strip
@@ -244,11 +260,26 @@ static_assert(kRowPrefetch >= 1 && kRowPrefetch <= 3, "row prefetch distance 1..
 // KP07_dimsplit even; the Euler kernel is 7 % (carry) and 20-25 % (pinned rows)
 // slower with them, so it keeps neither, and 1.5 % faster with one row ahead
 // instead of two.
+// Columns per lane, rows loaded ahead and row pinning of the KP07_dimsplit and
+// HLL2 marches (build-time choices; the defaults are the measured best).
+// Measured at 4096^2 per substep, same box: one column, 3 rows ahead, pinned
+// rows: KP07_dimsplit 113.0 us, HLL2 132.1; two columns (8-byte accesses):
+// 1 row ahead 111.3-112.7 and 156-159 us, 2 rows ahead 132-146 and 153-176 us.
+#ifndef FVMI_SWE_COLS
+#define FVMI_SWE_COLS 1
+#endif
+#ifndef FVMI_SWE_PREFETCH
+#define FVMI_SWE_PREFETCH 3
+#endif
+#ifndef FVMI_SWE_PIN
+#define FVMI_SWE_PIN 1
+#endif
+
 template <class M>
 struct SweBase {
     static constexpr int NV = 3;
     static constexpr bool kEuler = false;
-    static constexpr bool kPinRows = true;
+    static constexpr bool kPinRows = FVMI_SWE_PIN != 0;
     static constexpr bool kCarryDiff = true;
     static constexpr int kPrefetch = kRowPrefetch;
     __device__ static void pred_diff(const float QL[3], const float QR[3], const SubstepArgs& a, float D[3]) {
@@ -272,8 +303,9 @@ struct SweBase {
 
 template <class M>
 struct SweKP07 : SweBase<M> {
-    static constexpr int kPrefetch = 3;  // 0.7 % faster than 2 at 4096^2
+    static constexpr int kPrefetch = FVMI_SWE_PREFETCH;
     static constexpr bool kSlopes = true;
+    static constexpr int kC = FVMI_SWE_COLS;  // columns per lane
     __device__ static void flux(const float l[3], const float r[3], const SubstepArgs& a, float o[3]) {
         if constexpr (M::exact)
             swe_cu_flux<M>(l, r, a.g, o);
@@ -286,8 +318,9 @@ struct SweKP07 : SweBase<M> {
 
 template <class M>
 struct SweHLL2 : SweBase<M> {
-    static constexpr int kPrefetch = 3;  // 1.1 % faster than 2 at 4096^2
+    static constexpr int kPrefetch = FVMI_SWE_PREFETCH;
     static constexpr bool kSlopes = true;
+    static constexpr int kC = FVMI_SWE_COLS;  // columns per lane
     __device__ static void flux(const float l[3], const float r[3], const SubstepArgs& a, float o[3]) {
         if constexpr (M::exact)
             swe_hll_flux<M>(l, r, a.g, o);
@@ -303,12 +336,17 @@ struct SweWAF : SweBase<M> {
     // the flux is assembled by the march from shared pieces (waf_cell,
     // waf_hstar, waf_flux_core in fv_math.h)
     static constexpr bool kSlopes = false;
+    static constexpr int kC = 1;  // columns per lane (its march needs the registers of two)
 };
 
 template <class M>
 struct EulerKP07 {
     static constexpr int NV = 4;
     static constexpr bool kSlopes = true;
+    // columns per lane: two in fast arithmetic (8-byte accesses: 8192^2 step
+    // 1.263 -> 1.157 ms); the exact arithmetic's division sequences need the
+    // registers of the second column
+    static constexpr int kC = M::exact ? 1 : 2;
     static constexpr bool kEuler = true;
     static constexpr bool kPinRows = false;
     static constexpr bool kCarryDiff = false;
@@ -417,44 +455,80 @@ __device__ __forceinline__ void reconstruct(const float l[S::NV], const float c[
 template <class M>
 __device__ __forceinline__ float evolve(float q, float f_in, float f_out, float dt, float d, float dtd) {
     if constexpr (M::exact) {
-        return q + (f_in - f_out) * dt / d;
+        return q + M::div((f_in - f_out) * dt, d);
     } else {
         return q + (f_in - f_out) * dtd;
     }
 }
 
-// x sweep of one padded row held across the 64 lanes (valid on lanes 2..61).
+// Neighbours across a row held C columns per lane: column c of this lane's
+// left neighbour is column c - 1 here, or for c = 0 column C - 1 of lane - 1;
+// the right one column c + 1, or for c = C - 1 column 0 of lane + 1.
+template <int C, int NV>
+__device__ __forceinline__ float left_of(const float (&x)[C][NV], int c, int k) {
+    return c > 0 ? x[c - 1][k] : from_left(x[C - 1][k]);
+}
+template <int C, int NV>
+__device__ __forceinline__ float right_of(const float (&x)[C][NV], int c, int k) {
+    return c < C - 1 ? x[c + 1][k] : from_right(x[0][k]);
+}
+
+// x sweep of one padded row held across the 64 lanes, C columns per lane
+// (valid on the strip's output columns).
 template <class S, class M>
-__device__ __forceinline__ void x_sweep_row(float q[S::NV], const SubstepArgs& a) {
+__device__ __forceinline__ void x_sweep_row(float (&q)[S::kC][S::NV], const SubstepArgs& a) {
     constexpr int NV = S::NV;
-    float F[NV];
+    constexpr int C = S::kC;
+    float F[C][NV];
     if constexpr (S::kSlopes) {
-        float l[NV], r[NV], qbl[NV], qbr[NV], qbl_r[NV];
+        const float kk = M::exact ? M::div(a.dt, 2.0f * a.dx) : 0.5f * a.dtdx;
+        float qbl[C][NV], qbr[C][NV];
 #pragma unroll
-        for (int k = 0; k < NV; ++k) {
-            l[k] = from_left(q[k]);
-            r[k] = from_right(q[k]);
+        for (int c = 0; c < C; ++c) {
+            float l[NV], r[NV];
+#pragma unroll
+            for (int k = 0; k < NV; ++k) {
+                l[k] = left_of<C, NV>(q, c, k);
+                r[k] = right_of<C, NV>(q, c, k);
+            }
+            reconstruct<S, M>(l, q[c], r, a, kk, qbl[c], qbr[c]);
         }
-        const float kk = M::exact ? a.dt / (2.0f * a.dx) : 0.5f * a.dtdx;
-        reconstruct<S, M>(l, q, r, a, kk, qbl, qbr);
 #pragma unroll
-        for (int k = 0; k < NV; ++k) qbl_r[k] = from_right(qbl[k]);
-        S::flux_x(qbr, qbl_r, a, F);  // interface (c, c+1)
+        for (int c = 0; c < C; ++c) {
+            float qbl_r[NV];
+#pragma unroll
+            for (int k = 0; k < NV; ++k) qbl_r[k] = right_of<C, NV>(qbl, c, k);
+            S::flux_x(qbr[c], qbl_r, a, F[c]);  // interface (c, c+1)
+        }
     } else {
-        // WAF: each lane's cell quantities, and the h* and limiter denominators
-        // of its right interface, once; the neighbours' through DPP
-        const WafCell P = waf_cell<M>(q, a.g, a.sqrt_g);
-        const WafCell R = waf_shift_right(P);
-        const float hd = waf_hstar<M>(P, R, a.g, a.inv_g);
-        const WafDen D = waf_den<M>(P, R, hd);
-        const WafDen Dl = {from_left(D.d1), from_left(D.d2), from_left(D.d3)};
-        const WafDen Dr = {from_right(D.d1), from_right(D.d2), from_right(D.d3)};
-        waf_flux_core<M>(P, R, Dl, D, Dr, hd, a.g, a.hg, a.dx, a.dt, a.dtdx, F);
+        // WAF: each column's cell quantities, and the h* and limiter
+        // denominators of its right interface, once; the neighbours' through DPP
+        WafCell P[C], R[C];
+        float hd[C];
+        WafDen D[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) P[c] = waf_cell<M>(q[c], a.g, a.sqrt_g);
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            R[c] = c < C - 1 ? P[c + 1] : waf_shift_right(P[0]);
+            hd[c] = waf_hstar<M>(P[c], R[c], a.g, a.inv_g);
+            D[c] = waf_den<M>(P[c], R[c], hd[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const WafDen Dl = c > 0 ? D[c - 1]
+                                    : WafDen{from_left(D[C - 1].d1), from_left(D[C - 1].d2), from_left(D[C - 1].d3)};
+            const WafDen Dr = c < C - 1 ? D[c + 1] : WafDen{from_right(D[0].d1), from_right(D[0].d2), from_right(D[0].d3)};
+            waf_flux_core<M>(P[c], R[c], Dl, D[c], Dr, hd[c], a.g, a.hg, a.dx, a.dt, a.dtdx, F[c]);
+        }
     }
 #pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        const float Fl = from_left(F[k]);
-        q[k] = evolve<M>(q[k], Fl, F[k], a.dt, a.dx, a.dtdx);
+    for (int c = 0; c < C; ++c) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            const float Fl = left_of<C, NV>(F, c, k);
+            q[c][k] = evolve<M>(q[c][k], Fl, F[c][k], a.dt, a.dx, a.dtdx);
+        }
     }
 }
 
@@ -474,46 +548,84 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t field_rsrc(const void* base, u
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
 }
 
-// Per-wave constants of a strip.
+typedef unsigned fvmi_v2u __attribute__((ext_vector_type(2)));
+
+// Per-wave constants of a strip of C columns per lane.  Lane L holds padded
+// columns wstart + C L + c; halo_lanes<C>() lanes at each end hold the x halo.
+//
+// Loads (C = 2): one 8-byte load per field and row at pair_off; its words are
+// the lane's two columns in order, or reversed (pair_rev) where both columns
+// are mirrored ghost columns.  A lane whose two mapped columns are not such a
+// pair at an 8-byte offset (zero-gradient ghosts, a periodic or mirrored
+// ghost next to an interior column when nx is odd, the columns past the
+// domain) loads each column with a 4-byte load instead (fix_off); waves
+// without such a lane (wave_fix false) skip those loads.
+// Stores (C = 2): an 8-byte store of the two columns where both are written
+// and 8-byte aligned (store_off), else 4-byte stores of the written ones
+// (single_off; only in waves with such a lane, wave_single).
+template <int C>
 struct Strip {
     int lane;
-    int col;       // padded column of this lane
-    int row0;      // padded row of r = 0
-    int r_end;     // rows r in [0, r_end)
-    int lane_off;    // byte offset of this lane's column in a row (after the boundary map)
-    unsigned xsign;  // sign-bit mask of the x momentum (mirrored ghost column)
-    bool lane_out;   // lane writes its column
-    int store_off;   // byte offset of the column for writing lanes, kMaskedOffset for the others
+    int col[C];          // padded column of each column of this lane
+    int row0;            // padded row of r = 0
+    int r_end;           // rows r in [0, r_end)
+    int lane_off[C];     // byte offset of each column after the boundary map
+    unsigned xsign[C];   // sign-bit mask of the x momentum (mirrored ghost column)
+    bool lane_out[C];    // column written
+    int store_off;       // C = 1: the column, C = 2: the pair (8-byte store); kMaskedOffset if not
+    int pair_off;        // C = 2: offset of the 8-byte load (kMaskedOffset for fix lanes)
+    bool pair_rev;       // C = 2: the pair's words are (column 1, column 0)
+    bool lane_fix;       // C = 2: this lane loads its columns one by one
+    bool wave_fix;       // C = 2: some lane of the wave does
+    int fix_off[2];      // C = 2: the 4-byte loads of a fix lane (kMaskedOffset elsewhere)
+    int single_off[2];   // C = 2: 4-byte stores of written columns not in a full aligned pair
+    bool wave_single;    // C = 2: some lane of the wave has such a column
     RowMap ymap;
     __amdgpu_buffer_rsrc_t rin[4], rout[4];
 };
 
 // x mirrors flip the x momentum, y mirrors the y momentum
-template <int NV>
-__device__ __forceinline__ void apply_signs(const Strip& st, unsigned ysign, float* v) {
-    v[1] = __uint_as_float(__float_as_uint(v[1]) ^ st.xsign);
-    v[2] = __uint_as_float(__float_as_uint(v[2]) ^ ysign);
+template <int C, int NV>
+__device__ __forceinline__ void apply_signs(const Strip<C>& st, unsigned ysign, float (&v)[C][NV]) {
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        v[c][1] = __uint_as_float(__float_as_uint(v[c][1]) ^ st.xsign[c]);
+        v[c][2] = __uint_as_float(__float_as_uint(v[c][2]) ^ ysign);
+    }
 }
 
 // Issues the loads of padded row r and returns the sign-bit mask of its y
 // momentum.  Pinned: the raw words are returned and use_row applies the mirror
 // signs at the row's first use; otherwise they are applied here.
-template <int NV, bool Pinned>
-__device__ __forceinline__ unsigned load_row_raw(const SubstepArgs& a, const Strip& st, int r, float v[NV]) {
+template <int C, int NV, bool Pinned, bool Edge = false>
+__device__ __forceinline__ unsigned load_row_raw(const SubstepArgs& a, const Strip<C>& st, int r,
+                                                 float (&v)[C][NV]) {
     // Row offsets are wave-uniform (SGPR soffset); the lane adds its column
     // offset (VGPR voffset): no per-lane address arithmetic.
     unsigned ysign;
     const int row = map_row<2>(st.ymap, __builtin_amdgcn_readfirstlane(st.row0 + r), ysign);
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-        // the b32 buffer builtins move raw 32-bit words
-        unsigned x = __builtin_amdgcn_raw_buffer_load_b32(st.rin[k], st.lane_off, row * a.in_pitch[k], 0);
-        if constexpr (!Pinned) {
-            if (k == 1) x ^= st.xsign;
-            if (k == 2) x ^= ysign;
+        const int soff = row * a.in_pitch[k];
+        if constexpr (C == 1) {
+            // the b32 buffer builtins move raw 32-bit words
+            v[0][k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(st.rin[k], st.lane_off[0], soff, 0));
+        } else {
+            static_assert(C == 2, "one or two columns per lane");
+            const fvmi_v2u x = __builtin_amdgcn_raw_buffer_load_b64(st.rin[k], st.pair_off, soff, 0);
+            unsigned w0 = st.pair_rev ? x.y : x.x;
+            unsigned w1 = st.pair_rev ? x.x : x.y;
+            if constexpr (Edge) {
+                const unsigned f0 = __builtin_amdgcn_raw_buffer_load_b32(st.rin[k], st.fix_off[0], soff, 0);
+                const unsigned f1 = __builtin_amdgcn_raw_buffer_load_b32(st.rin[k], st.fix_off[1], soff, 0);
+                w0 = st.lane_fix ? f0 : w0;
+                w1 = st.lane_fix ? f1 : w1;
+            }
+            v[0][k] = __uint_as_float(w0);
+            v[1][k] = __uint_as_float(w1);
         }
-        v[k] = __uint_as_float(x);
     }
+    if constexpr (!Pinned) apply_signs<C, NV>(st, ysign, v);
     return ysign;
 }
 
@@ -523,186 +635,228 @@ __device__ __forceinline__ unsigned load_row_raw(const SubstepArgs& a, const Str
 // instruction scheduler cannot hoist the row's consumers (and the wait for its
 // data) above the later rows' loads, and each load keeps its distance of
 // kRowPrefetch rows of work.  The mirror signs are applied after it.
-template <int NV, bool Pinned>
-__device__ __forceinline__ void use_row(const Strip& st, unsigned ysign, float (&v)[NV]) {
+template <int C, int NV, bool Pinned>
+__device__ __forceinline__ void use_row(const Strip<C>& st, unsigned ysign, float (&v)[C][NV]) {
     if constexpr (Pinned) {
 #pragma unroll
-        for (int k = 0; k < NV; ++k) asm volatile("" : "+v"(v[k]));
-        apply_signs<NV>(st, ysign, v);
+        for (int c = 0; c < C; ++c)
+#pragma unroll
+            for (int k = 0; k < NV; ++k) asm volatile("" : "+v"(v[c][k]));
+        apply_signs<C, NV>(st, ysign, v);
     }
 }
 
-template <int NV>
-__device__ __forceinline__ void store_row(const SubstepArgs& a, const Strip& st, int r, const float (&q)[NV]) {
+template <int C, int NV, bool Edge = false>
+__device__ __forceinline__ void store_row(const SubstepArgs& a, const Strip<C>& st, int r, const float (&q)[C][NV]) {
     const int prow = __builtin_amdgcn_readfirstlane(st.row0 + r - 2);
     // every lane stores; the halo lanes' offset is out of range and the
     // hardware drops their writes (no branch around the stores)
 #pragma unroll
-    for (int k = 0; k < NV; ++k)
-        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(q[k]), st.rout[k], st.store_off, prow * a.out_pitch[k], 0);
+    for (int k = 0; k < NV; ++k) {
+        const int soff = prow * a.out_pitch[k];
+        if constexpr (C == 1) {
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(q[0][k]), st.rout[k], st.store_off, soff, 0);
+        } else {
+            const fvmi_v2u x = {__float_as_uint(q[0][k]), __float_as_uint(q[1][k])};
+            __builtin_amdgcn_raw_buffer_store_b64(x, st.rout[k], st.store_off, soff, 0);
+        }
+    }
+    if constexpr (C == 2 && Edge) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            const int soff = prow * a.out_pitch[k];
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(q[0][k]), st.rout[k], st.single_off[0], soff, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(q[1][k]), st.rout[k], st.single_off[1], soff, 0);
+        }
+    }
 }
 
 // CFL accumulator: the exact policy keeps min(dt) as the reference's writeCfl
 // does; the fast policy keeps max(1/dt) (one reciprocal per strip, taken in
 // finish_cfl).
 template <class S, class M>
-__device__ __forceinline__ void accumulate_cfl(const float (&q)[S::NV], const SubstepArgs& a, bool take, float& acc) {
-    if constexpr (M::exact) {
-        const float cd = S::cell_dt(q, a);
-        acc = take ? fminf(acc, cd) : acc;
-    } else {
-        const float rate = S::cell_rate(q, a);
-        acc = take ? fmaxf(acc, rate) : acc;
+__device__ __forceinline__ void accumulate_cfl(const float (&q)[S::kC][S::NV], const SubstepArgs& a,
+                                               const Strip<S::kC>& st, float& acc) {
+#pragma unroll
+    for (int c = 0; c < S::kC; ++c) {
+        if constexpr (M::exact) {
+            const float cd = S::cell_dt(q[c], a);
+            acc = st.lane_out[c] ? fminf(acc, cd) : acc;
+        } else {
+            const float rate = S::cell_rate(q[c], a);
+            acc = st.lane_out[c] ? fmaxf(acc, rate) : acc;
+        }
     }
 }
 
-// One row step of the y sweep for slope schemes.  Loop-carried state: qbt, the
-// predicted top face of row r-2, and g, the flux through its bottom interface;
-// both are replaced by the values of row r-1 (in place: reads precede writes).
-template <class S, class M, bool XFirst>
-__device__ __forceinline__ void slope_row(const SubstepArgs& a, const Strip& st, float kky,
-                                          const float (&w0)[S::NV], const float (&w1)[S::NV], float (&w2)[S::NV],
-                                          float (&qbt)[S::NV], float (&g)[S::NV], float (&dq)[S::NV], int r,
-                                          bool emit, float& dt_min) {
+// One row step of the y sweep for slope schemes.  Loop-carried state (per
+// column): qbt, the predicted top face of row r-2, and g, the flux through its
+// bottom interface; both are replaced by the values of row r-1 (in place:
+// reads precede writes).
+template <class S, class M, bool XFirst, bool Edge>
+__device__ __forceinline__ void slope_row(const SubstepArgs& a, const Strip<S::kC>& st, float kky,
+                                          const float (&w0)[S::kC][S::NV], const float (&w1)[S::kC][S::NV],
+                                          float (&w2)[S::kC][S::NV], float (&qbt)[S::kC][S::NV],
+                                          float (&g)[S::kC][S::NV], float (&dq)[S::kC][S::NV], int r, bool emit,
+                                          float& dt_min) {
     // w0, w1: rows r-2, r-1 (already x swept when XFirst); w2: raw row r on entry.
     // dq (fast policy): y-order difference row r-1 minus row r-2 on entry, row r
     // minus row r-1 on exit.
     constexpr int NV = S::NV;
+    constexpr int C = S::kC;
     if constexpr (XFirst) x_sweep_row<S, M>(w2, a);
-    float y0[NV], y1[NV], y2[NV], qbb[NV], qbt_next[NV];
-    to_y<NV>(w0, y0);
-    to_y<NV>(w1, y1);
-    to_y<NV>(w2, y2);
-    if constexpr (M::exact || !S::kCarryDiff) {
-        reconstruct<S, M>(y0, y1, y2, a, kky, qbb, qbt_next);  // faces of row r-1
-    } else {
-        float f[NV];
+    float outq[C][NV];
 #pragma unroll
-        for (int k = 0; k < NV; ++k) f[k] = y2[k] - y1[k];
-        reconstruct_fast<S>([&](int k) { return dq[k]; }, [&](int k) { return f[k]; }, y1, a, kky, qbb,
-                            qbt_next);  // faces of row r-1
+    for (int c = 0; c < C; ++c) {
+        float y0[NV], y1[NV], y2[NV], qbb[NV], qbt_next[NV];
+        to_y<NV>(w0[c], y0);
+        to_y<NV>(w1[c], y1);
+        to_y<NV>(w2[c], y2);
+        if constexpr (M::exact || !S::kCarryDiff) {
+            reconstruct<S, M>(y0, y1, y2, a, kky, qbb, qbt_next);  // faces of row r-1
+        } else {
+            float f[NV];
 #pragma unroll
-        for (int k = 0; k < NV; ++k) dq[k] = f[k];
+            for (int k = 0; k < NV; ++k) f[k] = y2[k] - y1[k];
+            reconstruct_fast<S>([&](int k) { return dq[c][k]; }, [&](int k) { return f[k]; }, y1, a, kky, qbb,
+                                qbt_next);  // faces of row r-1
+#pragma unroll
+            for (int k = 0; k < NV; ++k) dq[c][k] = f[k];
+        }
+        float G[NV];
+        S::flux_y(qbt[c], qbb, a, G);  // interface (r-2, r-1)
+        if (emit) {
+#pragma unroll
+            for (int k = 0; k < NV; ++k) y0[k] = evolve<M>(y0[k], g[c][k], G[k], a.dt, a.dy, a.dtdy);
+            to_y<NV>(y0, outq[c]);  // row r-2 after the y sweep
+        }
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            qbt[c][k] = qbt_next[k];
+            g[c][k] = G[k];
+        }
     }
-    float G[NV];
-    S::flux_y(qbt, qbb, a, G);  // interface (r-2, r-1)
     if (emit) {
-        float outq[NV];
-#pragma unroll
-        for (int k = 0; k < NV; ++k) y0[k] = evolve<M>(y0[k], g[k], G[k], a.dt, a.dy, a.dtdy);
-        to_y<NV>(y0, outq);  // row r-2 after the y sweep
         if constexpr (!XFirst) x_sweep_row<S, M>(outq, a);
         if constexpr (S::kEuler) {
             // gravity source, EE2D_KP07_dimsplit.cu:197-205
             if (a.g > 0.0f) {
-                const float rho_v = outq[2];
-                outq[2] -= a.g * outq[0] * a.dt;
-                outq[3] -= a.g * rho_v * a.dt;
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    const float rho_v = outq[c][2];
+                    outq[c][2] -= a.g * outq[c][0] * a.dt;
+                    outq[c][3] -= a.g * rho_v * a.dt;
+                }
             }
         }
-        store_row<NV>(a, st, r, outq);
+        store_row<C, NV, Edge>(a, st, r, outq);
         // the CFL minimum is taken after the y-then-x substep only: it is what
         // computeDt reads after a full step (the last launch's values)
-        if constexpr (!XFirst) accumulate_cfl<S, M>(outq, a, st.lane_out, dt_min);
-    }
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        qbt[k] = qbt_next[k];
-        g[k] = G[k];
+        if constexpr (!XFirst) accumulate_cfl<S, M>(outq, a, st, dt_min);
     }
 }
 
 // WAF: the y window holds 3 rows (r-2 .. r) and there is no slope state.
-// Each ring slot carries its row's WafCell (sweep-y view), computed once when
+// Each ring slot carries its row's WafCells (sweep-y view), computed once when
 // the row is ready; hd carries h* of the interface (r-2, r-1) and dd[0], dd[1]
 // the limiter denominators of the interfaces (r-3, r-2) and (r-2, r-1); g is
 // the flux through the bottom interface of row r-2.  All carried values are
 // replaced in place.
-template <class S, class M, bool XFirst>
-__device__ __forceinline__ void waf_row(const SubstepArgs& a, const Strip& st, const float (&w1)[3],
-                                        float (&w3)[3], const WafCell& c1, const WafCell& c2, WafCell& c3,
-                                        float& hd, WafDen (&dd)[2], float (&g)[3], int r, bool emit, float& dt_min) {
+template <class S, class M, bool XFirst, bool Edge>
+__device__ __forceinline__ void waf_row(const SubstepArgs& a, const Strip<S::kC>& st, const float (&w1)[S::kC][3],
+                                        float (&w3)[S::kC][3], const WafCell (&c1)[S::kC],
+                                        const WafCell (&c2)[S::kC], WafCell (&c3)[S::kC], float (&hd)[S::kC],
+                                        WafDen (&dd)[2][S::kC], float (&g)[S::kC][3], int r, bool emit,
+                                        float& dt_min) {
+    constexpr int C = S::kC;
     // w1: row r-2 (x swept when XFirst), w3: raw row r on entry
     if constexpr (XFirst) x_sweep_row<S, M>(w3, a);
-    {
-        float y3[3];
-        to_y<3>(w3, y3);
-        c3 = waf_cell<M>(y3, a.g, a.sqrt_g);
-    }
-    const float hd_r = waf_hstar<M>(c2, c3, a.g, a.inv_g);  // interface (r-1, r)
-    const WafDen Dn = waf_den<M>(c2, c3, hd_r);
-    float G[3];
-    waf_flux_core<M>(c1, c2, dd[0], dd[1], Dn, hd, a.g, a.hg, a.dy, a.dt, a.dtdy, G);  // interface (r-2, r-1)
-    if (emit) {
-        float y1[3], outq[3];
-        to_y<3>(w1, y1);
+    float outq[C][3];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) y1[k] = evolve<M>(y1[k], g[k], G[k], a.dt, a.dy, a.dtdy);
-        to_y<3>(y1, outq);
+    for (int c = 0; c < C; ++c) {
+        {
+            float y3[3];
+            to_y<3>(w3[c], y3);
+            c3[c] = waf_cell<M>(y3, a.g, a.sqrt_g);
+        }
+        const float hd_r = waf_hstar<M>(c2[c], c3[c], a.g, a.inv_g);  // interface (r-1, r)
+        const WafDen Dn = waf_den<M>(c2[c], c3[c], hd_r);
+        float G[3];
+        waf_flux_core<M>(c1[c], c2[c], dd[0][c], dd[1][c], Dn, hd[c], a.g, a.hg, a.dy, a.dt, a.dtdy,
+                         G);  // interface (r-2, r-1)
+        if (emit) {
+            float y1[3];
+            to_y<3>(w1[c], y1);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) y1[k] = evolve<M>(y1[k], g[c][k], G[k], a.dt, a.dy, a.dtdy);
+            to_y<3>(y1, outq[c]);
+        }
+        hd[c] = hd_r;
+        dd[0][c] = dd[1][c];
+        dd[1][c] = Dn;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) g[c][k] = G[k];
+    }
+    if (emit) {
         if constexpr (!XFirst) x_sweep_row<S, M>(outq, a);
-        store_row<3>(a, st, r, outq);
+        store_row<C, 3, Edge>(a, st, r, outq);
         // the CFL minimum is taken after the y-then-x substep only: it is what
         // computeDt reads after a full step (the last launch's values)
-        if constexpr (!XFirst) accumulate_cfl<S, M>(outq, a, st.lane_out, dt_min);
+        if constexpr (!XFirst) accumulate_cfl<S, M>(outq, a, st, dt_min);
     }
-    hd = hd_r;
-    dd[0] = dd[1];
-    dd[1] = Dn;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) g[k] = G[k];
-}
-
-template <int NV>
-__device__ __forceinline__ void copyv(float (&d)[NV], const float (&s)[NV]) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) d[k] = s[k];
 }
 
 // Row march of a slope scheme over one segment.  Row m of the segment lives in
 // ring slot m % N; the steady loop is unrolled N times so that every slot index
 // is a compile-time constant (the ring stays in registers, nothing is copied).
-template <class S, class M, bool XFirst>
-__device__ __forceinline__ float march_slopes(const SubstepArgs& a, const Strip& st) {
+// Edge: the wave has lanes whose columns need 4-byte loads or stores (see
+// Strip); the other waves' marches carry no code for them.
+template <class S, class M, bool XFirst, bool Edge>
+__device__ __forceinline__ float march_slopes(const SubstepArgs& a, const Strip<S::kC>& st) {
     constexpr int NV = S::NV;
+    constexpr int C = S::kC;
     constexpr int PF = S::kPrefetch;
     constexpr int N = PF + 3;  // rows r-2, r-1, r and PF rows in flight
-    const float kky = M::exact ? a.dt / (2.0f * a.dy) : 0.5f * a.dtdy;
+    const float kky = M::exact ? M::div(a.dt, 2.0f * a.dy) : 0.5f * a.dtdy;
     float dt_min = M::exact ? __builtin_inff() : 0.0f;
-    float ring[N][NV], P[NV], G[NV], dq[NV];
+    float ring[N][C][NV], P[C][NV], G[C][NV], dq[C][NV];
     unsigned ys[N];  // y-momentum sign mask of each slot's row
     const int last = st.r_end - 1;  // r_end >= 5
     // prologue: rows 0 .. N-1; faces of row 1; then step r = 3 (flux (1,2), no output)
 #pragma unroll
-    for (int m = 0; m < N; ++m) ys[m] = load_row_raw<NV, S::kPinRows>(a, st, min(m, last), ring[m]);
+    for (int m = 0; m < N; ++m) ys[m] = load_row_raw<C, NV, S::kPinRows, Edge>(a, st, min(m, last), ring[m]);
 #pragma unroll
-    for (int m = 0; m < 3; ++m) use_row<NV, S::kPinRows>(st, ys[m], ring[m]);
+    for (int m = 0; m < 3; ++m) use_row<C, NV, S::kPinRows>(st, ys[m], ring[m]);
     if constexpr (XFirst) {
         x_sweep_row<S, M>(ring[0], a);
         x_sweep_row<S, M>(ring[1], a);
         x_sweep_row<S, M>(ring[2], a);
     }
-    {
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
         float y0[NV], y1[NV], y2[NV], qbb[NV];
-        to_y<NV>(ring[0], y0);
-        to_y<NV>(ring[1], y1);
-        to_y<NV>(ring[2], y2);
-        reconstruct<S, M>(y0, y1, y2, a, kky, qbb, P);  // faces of row 1
+        to_y<NV>(ring[0][c], y0);
+        to_y<NV>(ring[1][c], y1);
+        to_y<NV>(ring[2][c], y2);
+        reconstruct<S, M>(y0, y1, y2, a, kky, qbb, P[c]);  // faces of row 1
 #pragma unroll
-        for (int k = 0; k < NV; ++k) dq[k] = y2[k] - y1[k];
+        for (int k = 0; k < NV; ++k) {
+            dq[c][k] = y2[k] - y1[k];
+            G[c][k] = 0.0f;
+        }
     }
-#pragma unroll
-    for (int k = 0; k < NV; ++k) G[k] = 0.0f;
-    ys[(3 + PF) % N] = load_row_raw<NV, S::kPinRows>(a, st, min(3 + PF, last), ring[(3 + PF) % N]);
-    use_row<NV, S::kPinRows>(st, ys[3 % N], ring[3 % N]);
-    slope_row<S, M, XFirst>(a, st, kky, ring[1], ring[2], ring[3 % N], P, G, dq, 3, false, dt_min);
+    ys[(3 + PF) % N] = load_row_raw<C, NV, S::kPinRows, Edge>(a, st, min(3 + PF, last), ring[(3 + PF) % N]);
+    use_row<C, NV, S::kPinRows>(st, ys[3 % N], ring[3 % N]);
+    slope_row<S, M, XFirst, Edge>(a, st, kky, ring[1], ring[2], ring[3 % N], P, G, dq, 3, false, dt_min);
     // steady state: r = 4, 4 + N, ...; the step of row rr loads row rr + PF into
     // the slot of row rr - 3, whose last use was the previous step.  Whole
     // groups of N rows run without branches; the remainder (< N rows) keeps
     // the same compile-time slots behind wave-uniform tests.
     auto step = [&](int u, int rr) {
-        ys[(4 + u + PF) % N] = load_row_raw<NV, S::kPinRows>(a, st, min(rr + PF, last), ring[(4 + u + PF) % N]);
-        use_row<NV, S::kPinRows>(st, ys[(4 + u) % N], ring[(4 + u) % N]);
-        slope_row<S, M, XFirst>(a, st, kky, ring[(2 + u) % N], ring[(3 + u) % N], ring[(4 + u) % N], P, G, dq, rr,
+        ys[(4 + u + PF) % N] =
+            load_row_raw<C, NV, S::kPinRows, Edge>(a, st, min(rr + PF, last), ring[(4 + u + PF) % N]);
+        use_row<C, NV, S::kPinRows>(st, ys[(4 + u) % N], ring[(4 + u) % N]);
+        slope_row<S, M, XFirst, Edge>(a, st, kky, ring[(2 + u) % N], ring[(3 + u) % N], ring[(4 + u) % N], P, G, dq, rr,
                                 true, dt_min);
     };
     int r = 4;
@@ -716,47 +870,52 @@ __device__ __forceinline__ float march_slopes(const SubstepArgs& a, const Strip&
     return dt_min;
 }
 
-// Row march of WAF (3-row window): row m and its WafCell in ring slot m % N.
-template <class S, class M, bool XFirst>
-__device__ __forceinline__ float march_waf(const SubstepArgs& a, const Strip& st) {
+// Row march of WAF (3-row window): row m and its WafCells in ring slot m % N.
+template <class S, class M, bool XFirst, bool Edge>
+__device__ __forceinline__ float march_waf(const SubstepArgs& a, const Strip<S::kC>& st) {
+    constexpr int C = S::kC;
     constexpr int PF = S::kPrefetch;
     constexpr int N = PF + 3;  // rows r-2 .. r and PF rows in flight
     float dt_min = M::exact ? __builtin_inff() : 0.0f;
-    float ring[N][3], G[3], hd;
-    WafCell cell[N];
-    WafDen dd[2];
+    float ring[N][C][3], G[C][3], hd[C];
+    WafCell cell[N][C];
+    WafDen dd[2][C];
     unsigned ys[N];  // y-momentum sign mask of each slot's row
     const int last = st.r_end - 1;  // r_end >= 5
     // prologue: rows 0 .. N-1; interfaces (0, 1) and (1, 2); row 3 + PF; then
     // step r = 3 (flux (1,2), no output)
 #pragma unroll
-    for (int m = 0; m < N; ++m) ys[m] = load_row_raw<3, S::kPinRows>(a, st, min(m, last), ring[m]);
+    for (int m = 0; m < N; ++m) ys[m] = load_row_raw<C, 3, S::kPinRows, Edge>(a, st, min(m, last), ring[m]);
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
-        use_row<3, S::kPinRows>(st, ys[m], ring[m]);
+        use_row<C, 3, S::kPinRows>(st, ys[m], ring[m]);
         if constexpr (XFirst) x_sweep_row<S, M>(ring[m], a);
-        float y[3];
-        to_y<3>(ring[m], y);
-        cell[m] = waf_cell<M>(y, a.g, a.sqrt_g);
-    }
-    {
-        const float hd01 = waf_hstar<M>(cell[0], cell[1], a.g, a.inv_g);
-        hd = waf_hstar<M>(cell[1], cell[2], a.g, a.inv_g);
-        dd[0] = waf_den<M>(cell[0], cell[1], hd01);
-        dd[1] = waf_den<M>(cell[1], cell[2], hd);
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            float y[3];
+            to_y<3>(ring[m][c], y);
+            cell[m][c] = waf_cell<M>(y, a.g, a.sqrt_g);
+        }
     }
 #pragma unroll
-    for (int k = 0; k < 3; ++k) G[k] = 0.0f;
+    for (int c = 0; c < C; ++c) {
+        const float hd01 = waf_hstar<M>(cell[0][c], cell[1][c], a.g, a.inv_g);
+        hd[c] = waf_hstar<M>(cell[1][c], cell[2][c], a.g, a.inv_g);
+        dd[0][c] = waf_den<M>(cell[0][c], cell[1][c], hd01);
+        dd[1][c] = waf_den<M>(cell[1][c], cell[2][c], hd[c]);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) G[c][k] = 0.0f;
+    }
     // row 3 + PF into the slot of row 0, whose last use was above
-    ys[(3 + PF) % N] = load_row_raw<3, S::kPinRows>(a, st, min(3 + PF, last), ring[(3 + PF) % N]);
-    use_row<3, S::kPinRows>(st, ys[3], ring[3]);
-    waf_row<S, M, XFirst>(a, st, ring[1], ring[3], cell[1], cell[2], cell[3], hd, dd, G, 3, false, dt_min);
+    ys[(3 + PF) % N] = load_row_raw<C, 3, S::kPinRows, Edge>(a, st, min(3 + PF, last), ring[(3 + PF) % N]);
+    use_row<C, 3, S::kPinRows>(st, ys[3], ring[3]);
+    waf_row<S, M, XFirst, Edge>(a, st, ring[1], ring[3], cell[1], cell[2], cell[3], hd, dd, G, 3, false, dt_min);
     // steady state: r = 4, 4 + N, ...; the step of row rr loads row rr + PF into
     // the slot of row rr - 3, whose last use was the previous step
     auto step = [&](int u, int rr) {
-        ys[(4 + u + PF) % N] = load_row_raw<3, S::kPinRows>(a, st, min(rr + PF, last), ring[(4 + u + PF) % N]);
-        use_row<3, S::kPinRows>(st, ys[(4 + u) % N], ring[(4 + u) % N]);
-        waf_row<S, M, XFirst>(a, st, ring[(2 + u) % N], ring[(4 + u) % N], cell[(2 + u) % N], cell[(3 + u) % N],
+        ys[(4 + u + PF) % N] = load_row_raw<C, 3, S::kPinRows, Edge>(a, st, min(rr + PF, last), ring[(4 + u + PF) % N]);
+        use_row<C, 3, S::kPinRows>(st, ys[(4 + u) % N], ring[(4 + u) % N]);
+        waf_row<S, M, XFirst, Edge>(a, st, ring[(2 + u) % N], ring[(4 + u) % N], cell[(2 + u) % N], cell[(3 + u) % N],
                               cell[(4 + u) % N], hd, dd, G, rr, true, dt_min);
     };
     int r = 4;
@@ -771,23 +930,65 @@ __device__ __forceinline__ float march_waf(const SubstepArgs& a, const Strip& st
 }
 
 template <class S, class M>
-__device__ __forceinline__ void setup_strip(const SubstepArgs& a, Strip& st, int lane, int strip_x, int row_in,
+__device__ __forceinline__ void setup_strip(const SubstepArgs& a, Strip<S::kC>& st, int lane, int strip_x, int row_in,
                                             int count) {
+    constexpr int C = S::kC;
+    constexpr int H = halo_lanes<C>();
     st.lane = lane;
-    st.col = a.x0 + kStripCols * strip_x + lane;
-    st.row0 = __builtin_amdgcn_readfirstlane(a.y0 + row_in);  // padded row of r = 0
-    st.r_end = __builtin_amdgcn_readfirstlane(count + 4);     // output rows r = 4 .. count+3
+    const int wstart = a.x0 + strip_cols<C>() * strip_x + 2 - H * C;  // padded column of lane 0, column 0
+    st.row0 = __builtin_amdgcn_readfirstlane(a.y0 + row_in);         // padded row of r = 0
+    st.r_end = __builtin_amdgcn_readfirstlane(count + 4);            // output rows r = 4 .. count+3
     st.ymap = make_row_map<2>(a.ny, (a.bc >> 16) & 0xF, (a.bc >> 24) & 0xF);
-    const AxisMap mx = map_axis(st.col, a.nx, a.bc & 0xF, (a.bc >> 8) & 0xF);
-    st.lane_off = mx.idx * (int)sizeof(float);
-    st.xsign = mx.mirrored ? 0x80000000u : 0u;
-    st.lane_out = lane >= 2 && lane < 2 + kStripCols && st.col < a.x1 + 2;
-    st.store_off = st.lane_out ? st.col * (int)sizeof(float) : kMaskedOffset;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        st.col[c] = wstart + C * lane + c;
+        const AxisMap mx = map_axis(st.col[c], a.nx, a.bc & 0xF, (a.bc >> 8) & 0xF);
+        st.lane_off[c] = mx.idx * (int)sizeof(float);
+        st.xsign[c] = mx.mirrored ? 0x80000000u : 0u;
+        st.lane_out[c] = lane >= H && lane < kLanes - H && st.col[c] < a.x1 + 2;
+    }
+    if constexpr (C == 1) {
+        st.store_off = st.lane_out[0] ? st.col[0] * (int)sizeof(float) : kMaskedOffset;
+    } else {
+        const int o0 = st.lane_off[0], o1 = st.lane_off[1];
+        const bool fwd = o1 == o0 + 4, rev = o1 == o0 - 4;
+        const int base = rev ? o1 : o0;
+        st.pair_rev = rev;
+        st.lane_fix = !(fwd || rev) || (base & 7) != 0;
+        st.pair_off = st.lane_fix ? kMaskedOffset : base;
+        st.fix_off[0] = st.lane_fix ? o0 : kMaskedOffset;
+        st.fix_off[1] = st.lane_fix ? o1 : kMaskedOffset;
+        st.wave_fix = __builtin_amdgcn_ballot_w64(st.lane_fix) != 0;
+        const int s0 = st.col[0] * (int)sizeof(float);
+        const bool full = st.lane_out[0] && st.lane_out[1] && (s0 & 7) == 0;
+        st.store_off = full ? s0 : kMaskedOffset;
+        st.single_off[0] = (!full && st.lane_out[0]) ? s0 : kMaskedOffset;
+        st.single_off[1] = (!full && st.lane_out[1]) ? s0 + 4 : kMaskedOffset;
+        st.wave_single = __builtin_amdgcn_ballot_w64(!full && (st.lane_out[0] || st.lane_out[1])) != 0;
+    }
 #pragma unroll
     for (int k = 0; k < S::NV; ++k) {
         st.rin[k] = field_rsrc(a.in[k], (unsigned)(a.ny + 4) * (unsigned)a.in_pitch[k]);
         st.rout[k] = field_rsrc(a.out[k], (unsigned)(a.ny + 4) * (unsigned)a.out_pitch[k]);
     }
+}
+
+// The march of one segment: sweep order by step, and the edge variant only
+// for waves that need it (a wave-uniform choice made once per segment).
+template <class S, class M>
+__device__ __forceinline__ float march(const SubstepArgs& a, const Strip<S::kC>& st) {
+    if constexpr (S::kC == 2) {
+        if (st.wave_fix || st.wave_single) {
+            if constexpr (S::kSlopes)
+                return (a.step == 0) ? march_slopes<S, M, true, true>(a, st) : march_slopes<S, M, false, true>(a, st);
+            else
+                return (a.step == 0) ? march_waf<S, M, true, true>(a, st) : march_waf<S, M, false, true>(a, st);
+        }
+    }
+    if constexpr (S::kSlopes)
+        return (a.step == 0) ? march_slopes<S, M, true, false>(a, st) : march_slopes<S, M, false, false>(a, st);
+    else
+        return (a.step == 0) ? march_waf<S, M, true, false>(a, st) : march_waf<S, M, false, false>(a, st);
 }
 
 // at least this many waves per SIMD must fit (the plan asks for four)
@@ -912,14 +1113,9 @@ __global__ void __launch_bounds__(256, FVMI_SUBSTEP_MIN_WAVES) substep_kernel(Su
     strip_x = __builtin_amdgcn_readfirstlane(strip_x);
     row_in = __builtin_amdgcn_readfirstlane(row_in);
     count = __builtin_amdgcn_readfirstlane(count);
-    Strip st;
+    Strip<S::kC> st;
     setup_strip<S, M>(a, st, lane, strip_x, row_in, count);
-    float acc;
-    if constexpr (S::kSlopes) {
-        acc = (a.step == 0) ? march_slopes<S, M, true>(a, st) : march_slopes<S, M, false>(a, st);
-    } else {
-        acc = (a.step == 0) ? march_waf<S, M, true>(a, st) : march_waf<S, M, false>(a, st);
-    }
+    const float acc = march<S, M>(a, st);
     float dt_min = acc;
     if constexpr (!M::exact) dt_min = (acc > 0.0f) ? fast::rcp(acc) : __builtin_inff();
     if (a.cfl != nullptr && a.step == 1) {

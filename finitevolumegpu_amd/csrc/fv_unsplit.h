@@ -80,7 +80,7 @@ __device__ __forceinline__ void lxf_2d_flux(const float ql[3], const float qr[3]
     float Fl[3], Fr[3];
     swe_F<M>(ql, g, Fl);
     swe_F<M>(qr, g, Fr);
-    if constexpr (M::exact) k = d / (4.0f * dt);
+    if constexpr (M::exact) k = M::div(d, 4.0f * dt);
 #pragma unroll
     for (int i = 0; i < 3; ++i) out[i] = 0.5f * (Fl[i] + Fr[i]) + k * (ql[i] - qr[i]);
 }
@@ -94,8 +94,8 @@ __device__ __forceinline__ void force_1d_flux(const float ql[3], const float qr[
     swe_F<M>(ql, g, Fl);
     swe_F<M>(qr, g, Fr);
     if constexpr (M::exact) {
-        klf = d / (2.0f * dt);
-        klw = dt / (2.0f * d);
+        klf = M::div(d, 2.0f * dt);
+        klw = M::div(dt, 2.0f * d);
     }
 #pragma unroll
     for (int i = 0; i < 3; ++i) q[i] = 0.5f * (ql[i] + qr[i]) + klw * (Fl[i] - Fr[i]);
@@ -286,7 +286,7 @@ __device__ __forceinline__ void swap_tn(const float (&q)[3], float (&o)[3]) {
 template <class M>
 __device__ __forceinline__ float u_update(float q, float fl, float fr, float gs, float gn, const UnsplitArgs& a) {
     if constexpr (M::exact)
-        return q + ((fl - fr) * a.dt / a.dx + (gs - gn) * a.dt / a.dy);
+        return q + (M::div((fl - fr) * a.dt, a.dx) + M::div((gs - gn) * a.dt, a.dy));
     else
         return q + ((fl - fr) * a.dtdx + (gs - gn) * a.dtdy);
 }

@@ -93,11 +93,19 @@ int run_substep(int scheme, SubstepArgs& a, int nvars, const fvmi_launch_t* laun
     if (!(a.dt >= 0.0f) || !std::isfinite(a.dt)) return fail(FVMI_ERR_INVALID, "dt must be finite and >= 0");
     if (!(a.dx > 0.0f) || !(a.dy > 0.0f)) return fail(FVMI_ERR_INVALID, "dx, dy must be > 0");
     const int min_pitch = (a.nx + 4) * (int)sizeof(float);
+    // schemes that march two columns per lane move them with 8-byte loads and
+    // stores: their fields and pitches must be 8-byte aligned
+    const int req_math = launch ? launch->math : FVMI_MATH_FAST;
+    const int align = scheme_cols_per_lane(scheme, req_math) == 2 ? 8 : 4;
     for (int k = 0; k < nvars; ++k) {
         if (!a.in[k] || !a.out[k]) return fail(FVMI_ERR_INVALID, "field pointer %d is NULL", k);
         if (a.in[k] == a.out[k]) return fail(FVMI_ERR_INVALID, "field %d: input and output alias", k);
-        if (a.in_pitch[k] < min_pitch || a.out_pitch[k] < min_pitch || (a.in_pitch[k] & 3) || (a.out_pitch[k] & 3))
-            return fail(FVMI_ERR_INVALID, "field %d: pitch must be >= %d bytes and a multiple of 4", k, min_pitch);
+        if (a.in_pitch[k] < min_pitch || a.out_pitch[k] < min_pitch || (a.in_pitch[k] % align) ||
+            (a.out_pitch[k] % align))
+            return fail(FVMI_ERR_INVALID, "field %d: pitch must be >= %d bytes and a multiple of %d", k, min_pitch,
+                        align);
+        if (((uintptr_t)a.in[k] % align) || ((uintptr_t)a.out[k] % align))
+            return fail(FVMI_ERR_INVALID, "field %d: pointers must be %d-byte aligned", k, align);
         // fields are addressed through buffer descriptors whose range must stay
         // below the 2 GiB offset that masked lanes' stores use
         if ((size_t)(a.ny + 4) * (size_t)a.in_pitch[k] >= ((size_t)1 << 31) ||
@@ -120,6 +128,7 @@ int run_substep(int scheme, SubstepArgs& a, int nvars, const fvmi_launch_t* laun
     a.dtdx = a.dt / a.dx;
     a.dtdy = a.dt / a.dy;
     set_slope_constants(a);
+    a.strip_cols = scheme_strip_cols(scheme, math);
     // WAF keeps equal segments: the round-weighted plan made it 3 % slower
     const int grid = plan_substep(a, current_cu_count(), scheme != SCHEME_WAF);
     hipError_t e = (math == FVMI_MATH_EXACT) ? launch_substep_exact(scheme, a, grid, (hipStream_t)stream)
@@ -739,6 +748,7 @@ int fvmi_substep_plan(int scheme, int nx, int ny, int x0, int y0, int x1, int y1
         return fail(FVMI_ERR_INVALID, "sub-area outside the domain");
     int math = 0;
     if (int rc = fill_launch(launch, a, &math)) return rc;
+    a.strip_cols = scheme_strip_cols(scheme, math);
     const int grid = plan_substep(a, cu_count, scheme != SCHEME_WAF);
     const long long total = (long long)grid * a.waves_per_block;
     if (total > (1LL << 30)) return fail(FVMI_ERR_INVALID, "plan too large");

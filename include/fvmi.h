@@ -18,6 +18,10 @@
  *     the layout of CudaArray2D (Common.py:502-531).  Pitches are in bytes, as
  *     the reference passes `strides[0]`.  Each field must be below 2 GiB,
  *     (ny+4)*pitch < 2^31 ((ny+2)*pitch for the one-ghost-cell schemes).
+ *     The kernel that marches two columns per lane (Euler in fast
+ *     arithmetic) moves them with 8-byte loads and stores: its field
+ *     pointers and pitches must be multiples of 8 bytes (hipMalloc /
+ *     fvmi_malloc_pitch give 256-byte alignment).
  *   - Boundary conditions are packed N<<24 | S<<16 | E<<8 | W with
  *     0 = Dirichlet, 2 = Periodic, 3 = Reflective (Simulator.py:44-92,
  *     common.h:109-130) and 4 = Outflow (zero-gradient ghost copy; an
@@ -256,7 +260,9 @@ typedef struct fvmi_instance {
 int fvmi_ensemble_prepare(fvmi_instance_t* table, int n, void* stream);
 /* One substep of every active instance over the work list `segs` (int4 per
  * segment: instance, strip index, first interior row, row count; strips are
- * 60 columns wide and a segment never crosses a strip).  The number of valid
+ * 124 columns wide for EE2D_KP07_dimsplit in fast arithmetic, which marches
+ * two columns per lane, and 60 for the others; a segment never crosses a
+ * strip).  The number of valid
  * segments is read on the device from *num_segs_dev, so a captured graph stays
  * valid when the host rewrites the list and its count between replays.
  * `waves` wavefronts are launched (<= 0: enough to fill the device); each
@@ -267,9 +273,9 @@ int fvmi_ensemble_substep(int scheme, fvmi_instance_t* table, const int* segs, c
                           int step, const fvmi_launch_t* launch, void* stream);
 /* The work split of one substep launch of the four dimensionally split entry
  * points and of the unsplit ones, computed on the host with no device call:
- * the planner's choice for scheme (0 KP07_dimsplit, 1 HLL2, 2 WAF, 3 Euler;
- * 4, 5, 6 LxF, HLL, FORCE, whose strips are 62 columns wide; 7 the unsplit
- * KP07, 60), sub-area [x0,x1)x[y0,y1) of an nx x ny domain (x1 = y1 = 0: the
+ * the planner's choice for scheme (3 Euler in fast arithmetic: strips of 124
+ * columns; 0 KP07_dimsplit, 1 HLL2, 2 WAF and 3 Euler in exact arithmetic: 60;
+ * 4, 5, 6 LxF, HLL, FORCE: 62; 7 the unsplit KP07: 60), sub-area [x0,x1)x[y0,y1) of an nx x ny domain (x1 = y1 = 0: the
  * whole domain, the only one the unsplit schemes take), the launch shape and
  * a device of `cu_count` CUs.  For every wave of the grid, 3 ints in `segs`
  * (capacity `max_waves` waves): strip, first row and row count relative to

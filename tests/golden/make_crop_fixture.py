@@ -2,7 +2,8 @@
 500 full steps of the 512x512 version of the 4096x4096 throughput workload
 (the dam break with 16 Gaussian bumps, radius and widths scaled by 1/8,
 reflective walls, dt = 0.45 x the initial CFL), together with the float64
-oracle's distance from it (relative L1 per field).  The GPU test of the crop
+oracle's distance from it (relative L1 per field), the step length dt and
+the CFL minimum after the last step (``cfl``).  The GPU test of the crop
 compares against this fixture instead of running the oracle, which takes
 minutes for WAF at this size.
 

@@ -141,6 +141,7 @@ def test_generated_ensemble_workload(ctx):
     ens = Ensemble(ctx, "kp07_dimsplit", specs, steps_per_graph=8, seg_rows=sched["seg_rows"],
                    waves_per_block=sched["waves_per_block"])
     ens.run()
+    ens.cell_steps()  # downloads the table
     nt = ens.tab["nt"][:len(specs)].copy()
     for i, spec in enumerate(specs):
         if i % 8 == 0:
@@ -161,6 +162,7 @@ def test_generated_ensemble_workload(ctx):
     spec = specs[j]
     ens = Ensemble(ctx, "kp07_dimsplit", [spec], math="exact", steps_per_graph=8)
     ens.run()
+    ens.cell_steps()  # downloads the table
     ora = O.OracleSimulator("kp07_dimsplit", spec["fields"], spec["nx"], spec["ny"], 1.0, 1.0, 9.81,
                             bc=_bc_code(spec["bc"]))
     ora.simulate(spec["t_end"])

@@ -47,23 +47,26 @@ def crop_reference(scheme):
     dt = np.float32(0.45 * float(ora.cfl))
     if scheme == "waf":
         with np.load(FIXTURE, allow_pickle=False) as z:
-            assert np.float32(z["cfl"]) == ora.cfl and int(z["steps"]) == CROP_STEPS and z["dt"] == dt
-            return (h, hu, hv), dt, [z["h"], z["hu"], z["hv"]], z["f32_vs_f64"].copy()
+            # dt = 0.45 x the initial CFL pins the initial data the fixture was made from
+            assert int(z["steps"]) == CROP_STEPS and np.float32(z["dt"]) == dt
+            return (h, hu, hv), dt, [z["h"], z["hu"], z["hv"]], z["f32_vs_f64"].copy(), np.float32(z["cfl"])
     for _ in range(CROP_STEPS):
         ora.step(dt)
-    return (h, hu, hv), dt, ora.download(), None
+    return (h, hu, hv), dt, ora.download(), None, np.float32(ora.cfl)
 
 
 @pytest.mark.parametrize("math", ["exact", "fast"])
 @pytest.mark.parametrize("scheme", SCHEMES)
 def test_crop_512_against_oracle(ctx, scheme, math):
     print("case crop 512", scheme, math, flush=True)
-    fields, dt, ref, d64 = crop_reference(scheme)
+    fields, dt, ref, d64, cfl_end = crop_reference(scheme)
     sim = make_gpu_sim(ctx, scheme, list(fields), 512, 512, 1.0, 1.0, WALLS, math=math)
     for _ in range(CROP_STEPS):
         sim.step(dt)
     got = sim.download()
     assert got[0].min() > 0.0
+    if math == "exact":  # the CFL minimum after the last step, bitwise
+        assert np.float32(sim.computeDt() * 2.0) == cfl_end
     for k in range(3):
         if math == "exact":
             assert np.array_equal(got[k], ref[k]), (k, float(np.abs(got[k] - ref[k]).max()))

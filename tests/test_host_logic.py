@@ -72,24 +72,24 @@ def test_simulate_fixed_dt():
 def test_ensemble_plan_segments_cover_each_row_once():
     """The vectorised work list equals the plain nested loop (instance, strip,
     row runs of `seg`, then a stable longest-first sort) and covers every row of
-    every 60-column strip exactly once."""
+    every strip (60 or 124 columns) exactly once."""
     from finitevolumegpu_amd.Ensemble import plan_segments
     rng = np.random.default_rng(4)
-    for seg in (1, 7, 24, 100):
+    for seg, width in ((1, 60), (7, 124), (24, 60), (100, 124)):
         ids = np.sort(rng.choice(50, size=13, replace=False))
         nx = rng.integers(2, 300, size=ids.size)
         ny = rng.integers(2, 300, size=ids.size)
-        got = plan_segments(ids, nx, ny, seg)
+        got = plan_segments(ids, nx, ny, seg, width)
         want = []
         for i, a, b in zip(ids, nx, ny):
-            for s in range(-(-int(a) // 60)):
+            for s in range(-(-int(a) // width)):
                 for r0 in range(0, int(b), seg):
                     want.append((int(i), s, r0, min(seg, int(b) - r0)))
         want.sort(key=lambda t: -t[3])
         assert got.dtype == np.int32 and got.shape == (len(want), 4)
         assert [tuple(r) for r in got.tolist()] == want
         for i, a, b in zip(ids, nx, ny):
-            for s in range(-(-int(a) // 60)):
+            for s in range(-(-int(a) // width)):
                 rows = np.zeros(int(b), dtype=int)
                 for _, _, r0, n in got[(got[:, 0] == i) & (got[:, 1] == s)]:
                     rows[r0:r0 + n] += 1

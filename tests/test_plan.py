@@ -1,7 +1,9 @@
 """The substep launch plan, checked on the host through fvmi_substep_plan
 (the same planner and segment mapping the kernels run; no device call): for
 every scheme, domain, sub-area, launch shape and CU count below, the waves'
-segments cover every row of every 60-column strip of the sub-area exactly once.
+segments cover every row of every strip of the sub-area exactly once (124
+columns for the schemes that march two columns per lane, 60 or 62 for the
+others).
 Also the choice between the round-weighted and the equal-segment plans and the
 wave budget of the equal-segment plan."""
 
@@ -13,12 +15,16 @@ import pytest
 from finitevolumegpu_amd import _lib
 
 SCHEMES = {"kp07": 0, "hll2": 1, "waf": 2, "ee2d": 3, "lxf": 4, "hll": 5, "force": 6, "kp07u": 7}
-STRIP = {"lxf": 62, "hll": 62, "force": 62}  # columns per strip (64 lanes less the ghost lanes)
+# output columns per strip: 64 lanes less the ghost lanes, times the columns per lane
+STRIP = {"kp07": 60, "hll2": 60, "ee2d": 124, "ee2d_exact": 60, "waf": 60, "lxf": 62, "hll": 62, "force": 62,
+         "kp07u": 60, "kp07_exact": 60}
 
 
 def plan(scheme, nx, ny, sub=(0, 0, 0, 0), tile_rows=0, waves_per_block=4, xcd_remap=True, cu_count=256):
     lib = _lib.lib()
-    launch = _lib.LaunchConfig(tile_rows, waves_per_block, _lib.FVMI_MATH_FAST, 1 if xcd_remap else 2)
+    math = _lib.FVMI_MATH_EXACT if scheme.endswith("_exact") else _lib.FVMI_MATH_FAST
+    scheme = scheme.replace("_exact", "")
+    launch = _lib.LaunchConfig(tile_rows, waves_per_block, math, 1 if xcd_remap else 2)
     n = ctypes.c_int()
     rc = lib.fvmi_substep_plan(SCHEMES[scheme], nx, ny, *sub, ctypes.byref(launch), cu_count, None, 0,
                                ctypes.byref(n))
@@ -51,6 +57,10 @@ CASES = [
     ("kp07", 6000, 777, (0, 0, 0, 0), 0, 4, True, 256),
     ("ee2d", 512, 8192, (2, 0, 510, 8192), 0, 4, True, 256),
     ("ee2d", 700, 900, (2, 2, 698, 898), 0, 4, True, 304),
+    ("ee2d_exact", 8192, 8192, (0, 0, 0, 0), 0, 4, True, 256),
+    ("ee2d_exact", 700, 900, (2, 2, 698, 898), 0, 4, True, 304),
+    ("ee2d", 701, 900, (699, 0, 701, 900), 0, 4, True, 256),
+    ("hll2", 127, 60, (0, 0, 0, 0), 0, 4, True, 256),
     ("kp07", 257, 129, (0, 0, 0, 0), 0, 4, True, 256),
     ("kp07", 13, 11, (0, 0, 0, 0), 0, 4, True, 256),
     ("kp07", 4096, 4096, (0, 0, 0, 0), 70, 4, True, 256),
@@ -73,14 +83,16 @@ def test_segments_cover_the_sub_area_once(scheme, nx, ny, sub, tr, wpb, remap, c
     x0, y0, x1, y1 = sub
     x1 = x1 or nx
     y1 = y1 or ny
-    assert_covers(segs, x1 - x0, y1 - y0, STRIP.get(scheme, 60))
+    assert_covers(segs, x1 - x0, y1 - y0, STRIP[scheme])
 
 
 def test_round_weighted_plan_at_4096():
     segs = plan("kp07", 4096, 4096)
     assert len(segs) == 4 * 256 * 4  # 4 rounds of one 4-wave workgroup per CU
     active = segs[segs[:, 0] >= 0]
-    assert len(active) == 69 * 14 * 4  # 14 row groups per strip, 4 parts each
+    # 69 strips of 60 columns in 18 bands of 4 (the last band three quarters
+    # full); 256 CUs / 18 bands = 14 row groups per band, 4 parts each
+    assert len(active) == 69 * 14 * 4
     # workgroup p is in round (p // 8) // 32; earlier rounds take longer parts (13:11:9:7)
     blocks = np.arange(len(segs)) // 4
     rounds = (blocks // 8) // 32
@@ -90,6 +102,21 @@ def test_round_weighted_plan_at_4096():
     first = sorted((r, c) for s, r, c in active if s == 0 and r < 293)
     assert [c for _, c in first] == [293 * 13 // 40, 293 * 11 // 40, 293 * 9 // 40,
                                      293 - 293 * 13 // 40 - 293 * 11 // 40 - 293 * 9 // 40] == [95, 80, 65, 53]
+    # the 4 waves of a workgroup take the same rows of 4 adjacent strips
+    for b in range(0, len(segs), 4):
+        grp = segs[b:b + 4]
+        live = grp[grp[:, 0] >= 0]
+        if len(live) > 1:
+            assert (live[:, 1] == live[0, 1]).all() and (live[:, 2] == live[0, 2]).all()
+            assert (np.diff(live[:, 0]) == 1).all() and live[0, 0] % 4 == 0
+
+
+def test_euler_fast_plan_uses_two_column_strips():
+    """The fast Euler kernel marches two columns per lane: 8192 columns in 67
+    strips of 124 (the exact one in 137 of 60)."""
+    fast = plan("ee2d", 8192, 8192)
+    exact = plan("ee2d_exact", 8192, 8192)
+    assert fast[:, 0].max() == 66 and exact[:, 0].max() == 136
 
 
 def test_waf_keeps_equal_segments():
@@ -100,7 +127,7 @@ def test_waf_keeps_equal_segments():
 
 def test_equal_plan_stays_within_four_waves_per_simd():
     # 137 strips at 8192 columns: the equal plan must not exceed 4 waves per SIMD
-    segs = plan("ee2d", 8192, 8192, tile_rows=0, waves_per_block=2)
+    segs = plan("ee2d_exact", 8192, 8192, tile_rows=0, waves_per_block=2)
     assert (segs[:, 0] >= 0).sum() <= 256 * 4 * 4
 
 
