@@ -2,7 +2,7 @@
 """Benchmark: KP07_dimsplit shallow water, 4096x4096 float32, on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--schemes kp07,hll2,waf,ee2d,lxf,hll,force,kp07u]
-                    [--no-cpu-baseline] [--ensemble 512] [--serve 2000 --serve-rate 200]
+                    [--no-cpu-baseline] [--ensemble 512] [--serve 2000 --serve-rate 300]
 
 A "step" is one full time step of the plain Simulator API (two fused substep
 launches: x-y then y-x sweeps), on the seeded dam break with 16 Gaussian bumps,
@@ -58,7 +58,8 @@ def _parse():
     ap.add_argument("--ensemble", type=int, default=512,
                     help="instances of the batched-ensemble leg (0 = skip the leg)")
     ap.add_argument("--serve", type=int, default=2000, help="requests of the serving leg (0 = skip the leg)")
-    ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-rate", type=float, default=300.0,
+                    help="Poisson arrival rate, requests/s (default just above the measured capacity, ~267/s)")
     ap.add_argument("--sequential", type=int, default=-1,
                     help="requests of the sequential serving baseline (-1 = all; the first K by arrival)")
     return ap.parse_args()
