@@ -160,3 +160,24 @@ def test_field_stats_validation():
     assert lib.fvmi_field_stats(126, 8, -1, 1, f, 1024, out, out, None) < 0
     assert lib.fvmi_field_stats(0, 8, 1, 1, f, 1024, out, out, None) < 0
     assert lib.fvmi_field_stats(126, 8, 1, 1, None, 1024, out, out, None) < 0
+
+
+def test_fast_euler_needs_8_byte_aligned_fields():
+    """The fast Euler kernel moves two columns per lane with 8-byte accesses:
+    a field pointer or pitch that is only 4-byte aligned is rejected before any
+    device work (no call here passes validation: the pointers are not device
+    memory)."""
+    lib = _lib.lib()
+    base = [0x100000 + 0x10000 * k for k in range(8)]
+
+    def euler(ptrs, pitch, math):
+        launch = _lib.LaunchConfig(0, 4, math, 1)
+        args = []
+        for p in ptrs:
+            args += [p, pitch]
+        return lib.fvmi_euler_kp07_dimsplit_substep(8, 8, 1.0, 1.0, 0.1, 0.1, 1.4, 1.3, 0, 0, *args, None,
+                                                    0, 0, 0, 0, ctypes.byref(launch), None)
+
+    odd_ptr = [base[0] + 4] + base[1:]
+    assert euler(odd_ptr, 64, _lib.FVMI_MATH_FAST) == -1 and "aligned" in _err()
+    assert euler(base, 52, _lib.FVMI_MATH_FAST) == -1 and "multiple of 8" in _err()
