@@ -1007,7 +1007,13 @@ __device__ __forceinline__ float march(const SubstepArgs& a, const Strip<S::kC>&
 // opened once per row run instead of once per strip); consecutive CUs of an
 // XCD take consecutive row groups of the same strips.
 constexpr int kRounds = 4;
-constexpr int kRoundWeights[kRounds] = {13, 11, 9, 7};
+// With the banded mapping, per launch against 13:11:9:7 (same box): KP07_dimsplit
+// 4096^2 113.5 / 114.1 us, HLL2 132.8 / 134.1, EE2D 8192^2 step 1.167 / 1.187 ms
+// (fast) and 2.125 / 2.156 (exact); 12:11:10:9 and 10:10:10:10 were slower.
+#ifndef FVMI_ROUND_WEIGHTS
+#define FVMI_ROUND_WEIGHTS 14, 11, 9, 6
+#endif
+constexpr int kRoundWeights[kRounds] = {FVMI_ROUND_WEIGHTS};
 constexpr int round_weight_sum() {
     int sum = 0;
     for (int k = 0; k < kRounds; ++k) sum += kRoundWeights[k];

@@ -93,15 +93,15 @@ def test_round_weighted_plan_at_4096():
     # 69 strips of 60 columns in 18 bands of 4 (the last band three quarters
     # full); 256 CUs / 18 bands = 14 row groups per band, 4 parts each
     assert len(active) == 69 * 14 * 4
-    # workgroup p is in round (p // 8) // 32; earlier rounds take longer parts (13:11:9:7)
+    # workgroup p is in round (p // 8) // 32; earlier rounds take longer parts (14:11:9:6)
     blocks = np.arange(len(segs)) // 4
     rounds = (blocks // 8) // 32
     mean = [segs[(rounds == k) & (segs[:, 0] >= 0), 2].mean() for k in range(4)]
     assert mean[0] > mean[1] > mean[2] > mean[3]
-    # a 293-row group (4096 rows in 14 groups) cut 13:11:9:7, the last part taking the remainder
+    # a 293-row group (4096 rows in 14 groups) cut 14:11:9:6, the last part taking the remainder
     first = sorted((r, c) for s, r, c in active if s == 0 and r < 293)
-    assert [c for _, c in first] == [293 * 13 // 40, 293 * 11 // 40, 293 * 9 // 40,
-                                     293 - 293 * 13 // 40 - 293 * 11 // 40 - 293 * 9 // 40] == [95, 80, 65, 53]
+    assert [c for _, c in first] == [293 * 14 // 40, 293 * 11 // 40, 293 * 9 // 40,
+                                     293 - 293 * 14 // 40 - 293 * 11 // 40 - 293 * 9 // 40] == [102, 80, 65, 46]
     # the 4 waves of a workgroup take the same rows of 4 adjacent strips
     for b in range(0, len(segs), 4):
         grp = segs[b:b + 4]
